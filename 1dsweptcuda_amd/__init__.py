@@ -1,0 +1,226 @@
+"""1dsweptcuda_amd — MI355X-native 1D swept-rule solver (heat, KS, Euler).
+
+Python front end over the C ABI of ``lib/libswept1d.so`` (declared in
+``include/swept1d.h``).  The compute path is the HIP library; this module only
+moves numpy arrays across the boundary and raises on every error code.  There
+is no CPU fallback: if the library is not built, or a GPU call fails, the call
+raises.
+
+The package directory name starts with a digit, so import it with
+``importlib.import_module("1dsweptcuda_amd")``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libswept1d.so")
+BIN_DIR = os.path.join(_HERE, "bin")
+HEADER = os.path.join(os.path.dirname(_HERE), "include", "swept1d.h")
+
+HEAT, KS, EULER = 0, 1, 2
+F64, F32 = 0, 1
+CLASSIC, SWEPT, ALT = 0, 1, 2
+BC_DEFAULT, BC_MIRROR, BC_PERIODIC, BC_DIRICHLET = 0, 1, 2, 3
+
+# sw1d_schedule kinds (Heat / KS / Euler classic and swept bookkeeping).
+SCHED_HEAT_CLASSIC, SCHED_HEAT_SWEPT = 0, 1
+SCHED_KS_CLASSIC, SCHED_KS_SWEPT, SCHED_KS_REGISTER = 2, 3, 4
+SCHED_EULER_CLASSIC, SCHED_EULER_SWEPT = 5, 6
+
+# sw1d_kernel_stats kinds
+K_CLASSIC, K_UP, K_DOWN, K_DIAMOND, K_SPLIT = 0, 1, 2, 3, 4
+
+
+class SweptError(RuntimeError):
+    """A libswept1d call returned a negative status."""
+
+    def __init__(self, code: int, where: str, message: str):
+        super().__init__(f"{where} failed ({code}): {message}")
+        self.code = code
+
+
+class Problem(C.Structure):
+    _fields_ = [
+        ("equation", C.c_int32),
+        ("precision", C.c_int32),
+        ("algorithm", C.c_int32),
+        ("bc", C.c_int32),
+        ("nx", C.c_int64),
+        ("tpb", C.c_int32),
+        ("node_width", C.c_int32),
+        ("dt", C.c_double),
+        ("dx", C.c_double),
+        ("alpha", C.c_double),
+    ]
+
+
+_lib = None
+
+
+def build() -> None:
+    """Compile libswept1d.so and the six programs in-tree (gfx950)."""
+    subprocess.run(["make", "-s", "-C", _HERE, "-j8"], check=True)
+
+
+def lib():
+    """Load libswept1d.so; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SweptError(-5, "load", f"{LIB_PATH} is not built (run build())")
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        L.sw1d_abi_version.restype = C.c_int
+        L.sw1d_last_error.restype = C.c_char_p
+        L.sw1d_device_check.restype = C.c_int
+        L.sw1d_plan_create.argtypes = [C.POINTER(P), C.POINTER(Problem)]
+        L.sw1d_plan_destroy.argtypes = [P]
+        L.sw1d_set_state.argtypes = [P, P]
+        L.sw1d_get_state.argtypes = [P, P]
+        L.sw1d_set_state_device.argtypes = [P, P]
+        L.sw1d_get_state_device.argtypes = [P, P]
+        L.sw1d_advance.argtypes = [P, C.c_int64]
+        L.sw1d_synchronize.argtypes = [P]
+        L.sw1d_advance_timed.argtypes = [P, C.c_int64, C.POINTER(C.c_double)]
+        L.sw1d_plan_info.argtypes = [P, C.POINTER(C.c_int64)]
+        L.sw1d_plan_constants.argtypes = [P, C.POINTER(C.c_double), C.c_int32]
+        L.sw1d_plan_stream.argtypes = [P, C.POINTER(P)]
+        L.sw1d_set_profiling.argtypes = [P, C.c_int32]
+        L.sw1d_kernel_stats.argtypes = [P, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+        L.sw1d_schedule.argtypes = [C.c_int32, C.c_double, C.c_double, C.c_double, C.c_int32,
+                                    C.c_double, C.POINTER(C.c_int64), C.POINTER(C.c_double),
+                                    C.c_int64, C.POINTER(C.c_int64)]
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, where: str) -> int:
+    if rc < 0:
+        raise SweptError(rc, where, lib().sw1d_last_error().decode())
+    return rc
+
+
+def last_error() -> str:
+    return lib().sw1d_last_error().decode()
+
+
+def device_check() -> bool:
+    return _check(lib().sw1d_device_check(), "sw1d_device_check") == 1
+
+
+def schedule(kind: int, dt: float, tf_arg: float, freq: float, tpb: int,
+             dt_sched: float | None = None):
+    """Rows the reference program writes: (steps computed, time printed)."""
+    if dt_sched is None:
+        dt_sched = dt
+    count = C.c_int64(0)
+    L = lib()
+    L.sw1d_schedule(kind, dt, tf_arg, freq, tpb, dt_sched, None, None, 0, C.byref(count))
+    steps = (C.c_int64 * count.value)()
+    times = (C.c_double * count.value)()
+    _check(L.sw1d_schedule(kind, dt, tf_arg, freq, tpb, dt_sched, steps, times, count.value,
+                           C.byref(count)), "sw1d_schedule")
+    return list(steps), list(times)
+
+
+def _dtype(precision: int):
+    return np.float64 if precision == F64 else np.float32
+
+
+class Plan:
+    """One solver instance on the current GPU (owns device state + a stream)."""
+
+    def __init__(self, equation: int, nx: int, dt: float, *, precision: int = F64,
+                 algorithm: int = SWEPT, tpb: int = 64, node_width: int = 0,
+                 bc: int = BC_DEFAULT, dx: float = 0.0, alpha: float = 0.0):
+        self.prob = Problem(equation, precision, algorithm, bc, nx, tpb, node_width, dt, dx, alpha)
+        self.dtype = _dtype(precision)
+        self.equation = equation
+        self.nx = nx
+        self._h = C.c_void_p()
+        _check(lib().sw1d_plan_create(C.byref(self._h), C.byref(self.prob)), "sw1d_plan_create")
+
+    @property
+    def state_len(self) -> int:
+        return self.nx * (3 if self.equation == EULER else 1)
+
+    def close(self):
+        if self._h:
+            lib().sw1d_plan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, arr) -> None:
+        a = np.ascontiguousarray(arr, dtype=self.dtype)
+        if a.size != self.state_len:
+            raise ValueError(f"state must hold {self.state_len} values, got {a.size}")
+        _check(lib().sw1d_set_state(self._h, a.ctypes.data_as(C.c_void_p)), "sw1d_set_state")
+
+    def get_state(self) -> np.ndarray:
+        out = np.empty(self.state_len, self.dtype)
+        _check(lib().sw1d_get_state(self._h, out.ctypes.data_as(C.c_void_p)), "sw1d_get_state")
+        return out
+
+    def set_state_device(self, ptr: int) -> None:
+        _check(lib().sw1d_set_state_device(self._h, C.c_void_p(ptr)), "sw1d_set_state_device")
+
+    def get_state_device(self, ptr: int) -> None:
+        _check(lib().sw1d_get_state_device(self._h, C.c_void_p(ptr)), "sw1d_get_state_device")
+
+    def advance(self, nsteps: int) -> None:
+        _check(lib().sw1d_advance(self._h, nsteps), "sw1d_advance")
+
+    def synchronize(self) -> None:
+        _check(lib().sw1d_synchronize(self._h), "sw1d_synchronize")
+
+    def advance_timed(self, nsteps: int) -> float:
+        ms = C.c_double(0)
+        _check(lib().sw1d_advance_timed(self._h, nsteps, C.byref(ms)), "sw1d_advance_timed")
+        return ms.value
+
+    def info(self) -> dict:
+        v = (C.c_int64 * 5)()
+        _check(lib().sw1d_plan_info(self._h, v), "sw1d_plan_info")
+        return {"node_width": v[0], "tile_steps": v[1], "values_per_point": v[2],
+                "device_bytes": v[3], "cycle_steps": v[4]}
+
+    def constants(self) -> list:
+        v = (C.c_double * 16)()
+        k = _check(lib().sw1d_plan_constants(self._h, v, 16), "sw1d_plan_constants")
+        return list(v[:k])
+
+    def stream(self) -> int:
+        s = C.c_void_p()
+        _check(lib().sw1d_plan_stream(self._h, C.byref(s)), "sw1d_plan_stream")
+        return s.value or 0
+
+    def set_profiling(self, on: bool) -> None:
+        _check(lib().sw1d_set_profiling(self._h, 1 if on else 0), "sw1d_set_profiling")
+
+    def kernel_stats(self, kind: int):
+        n = C.c_int64(0)
+        ms = C.c_double(0)
+        _check(lib().sw1d_kernel_stats(self._h, kind, C.byref(n), C.byref(ms)), "sw1d_kernel_stats")
+        return n.value, ms.value
+
+
+def program(equation: int, precision: int) -> str:
+    """Path of the drop-in executable, e.g. bin/KSDoubleOut."""
+    name = ["Heat", "KS", "Euler"][equation] + ("Double" if precision == F64 else "Single") + "Out"
+    return os.path.join(BIN_DIR, name)

@@ -1,0 +1,329 @@
+// Drop-in command line programs {Heat,KS,Euler}{Single,Double}Out.
+//
+// Same argv, messages, .dat solution file and .txt timing line as the
+// reference's main() (Heat1D_SweptShared.cu:697-836, KS1D_SweptShared.cu:
+// 866-1029, Euler1D_SweptShared.cu:1112-1287), on top of the libswept1d C ABI.
+// Built once per equation (SW1D_CLI_EQ = 0 heat, 1 KS, 2 Euler) and precision
+// (SW1D_CLI_F64 = 1 double, 0 float), like the six reference binaries
+// (SweptSource/Makefile:17-35).
+//
+//   <prog> nX tpb dt tf freq alg solution.dat [timing.txt]
+//
+// The reference's time bookkeeping is replayed by sw1d_schedule(); the
+// library advances the state by the resulting step counts.
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <iostream>
+#include <vector>
+
+#include "../../include/swept1d.h"
+
+#ifndef SW1D_CLI_EQ
+#error "SW1D_CLI_EQ must be defined (0 heat, 1 KS, 2 Euler)"
+#endif
+#ifndef SW1D_CLI_F64
+#error "SW1D_CLI_F64 must be defined (1 double, 0 float)"
+#endif
+
+#if SW1D_CLI_F64
+typedef double REAL;
+#else
+typedef float REAL;
+#endif
+
+using namespace std;
+
+namespace {
+
+[[noreturn]] void die_lib(const char* what)
+{
+    cout << what << ": " << sw1d_last_error() << endl;
+    exit(-1);
+}
+
+const char* prec_name() { return (sizeof(REAL) < 6) ? "Single" : "Double"; }
+
+struct Rows {
+    vector<int64_t> steps;
+    vector<double> times;
+};
+
+Rows schedule(int kind, double dt, double tf_arg, double freq, int tpb, double dt_sched)
+{
+    int64_t count = 0;
+    sw1d_schedule(kind, dt, tf_arg, freq, tpb, dt_sched, nullptr, nullptr, 0, &count);
+    Rows r;
+    r.steps.resize((size_t)count);
+    r.times.resize((size_t)count);
+    if (sw1d_schedule(kind, dt, tf_arg, freq, tpb, dt_sched, r.steps.data(), r.times.data(), count,
+                      &count) != SW1D_OK)
+        die_lib("schedule");
+    return r;
+}
+
+void usage_exit()
+{
+#if SW1D_CLI_EQ == 2
+    cout << "The Program takes 8 inputs, #Divisions, #Threads/block, deltat, finish time, output frequency..." << endl;
+#else
+    cout << "The Program takes 9 inputs, #Divisions, #Threads/block, deltat, finish time, output frequency..." << endl;
+#endif
+    cout << "Algorithm type, Variable Output File, Timing Output File (optional)" << endl;
+    exit(-1);
+}
+
+void invalid_exit()
+{
+    cout << "INVALID NUMERIC INPUT!! " << endl;
+    cout << "2nd ARGUMENT MUST BE A POWER OF TWO >= 32 AND FIRST ARGUMENT MUST BE DIVISIBLE BY SECOND" << endl;
+    exit(-1);
+}
+
+#if SW1D_CLI_EQ == 2
+// Euler:752-759 (energy) and :183-193 (pressure) in REAL arithmetic.
+REAL energy_of(const REAL* c)
+{
+    const REAL u = c[1] / c[0];
+    return c[2] / c[0] - REAL(0.5) * u * u;
+}
+REAL pressure_of(const REAL* c, REAL mgam) { return mgam * (c[2] - (REAL(0.5) * c[1] * c[1] / c[0])); }
+
+void euler_rows(ofstream& f, double t, const vector<REAL>& q, int dv, REAL mgam, bool initial)
+{
+    f << "Density " << t << " ";
+    for (int k = 1; k < dv - 1; k++) f << q[3 * k] << " ";
+    f << endl;
+    f << "Velocity " << t << " ";
+    // the t = 0 rows print rho*u and E/rho (Euler:1194-1200)
+    for (int k = 1; k < dv - 1; k++) f << (initial ? q[3 * k + 1] : q[3 * k + 1] / q[3 * k]) << " ";
+    f << endl;
+    f << "Energy " << t << " ";
+    for (int k = 1; k < dv - 1; k++) f << (initial ? q[3 * k + 2] / q[3 * k] : energy_of(&q[3 * k])) << " ";
+    f << endl;
+    f << "Pressure " << t << " ";
+    for (int k = 1; k < dv - 1; k++) f << pressure_of(&q[3 * k], mgam) << " ";
+    f << endl;
+}
+#endif
+
+}  // namespace
+
+int main(int argc, char* argv[])
+{
+    if (argc < 8) usage_exit();
+    cout.precision(10);
+
+    const int dv = atoi(argv[1]);
+    const int tpb = atoi(argv[2]);
+    const double dt = atof(argv[3]);
+    [[maybe_unused]] const double tf = atof(argv[4]) - 0.25 * dt;
+    const double freq = atof(argv[5]);
+    const int scheme = atoi(argv[6]);
+    const int bks = tpb != 0 ? dv / tpb : 0;
+
+    sw1d_problem prob{};
+    prob.precision = SW1D_CLI_F64 ? SW1D_F64 : SW1D_F32;
+    prob.nx = dv;
+    prob.tpb = tpb;
+    prob.dt = dt;
+    prob.algorithm = scheme == 0 ? SW1D_CLASSIC : (scheme == 1 ? SW1D_SWEPT : SW1D_ALT);
+
+    vector<REAL> IC;
+    int sched_kind = 0;
+    double dt_sched = dt;
+    [[maybe_unused]] const char* row_tag = "";
+    [[maybe_unused]] REAL mgam = REAL(0.4);
+
+#if SW1D_CLI_EQ == 0
+    // Heat1D_SweptShared.cu:56-58, 719-730, 751-754.
+    prob.equation = SW1D_HEAT;
+    const REAL th_diff = (REAL)8.418e-5;
+    const REAL ds = (REAL)0.001;
+    const double lx = ds * ((double)dv - 1.0);
+    const double fou = th_diff * dt / (ds * ds);
+    cout << "Heat --- #Blocks: " << bks << " | Length: " << lx << " | Precision: " << prec_name()
+         << " | Fo: " << fou << endl;
+    if ((dv & ((tpb - 1) != 0)) || (tpb & 31) != 0) invalid_exit();  // Heat:735 as evaluated
+    IC.resize((size_t)dv);
+    const REAL lxr = (REAL)lx;
+    for (int k = 0; k < dv; k++) {
+        const REAL a = ((REAL)k * ds);
+        IC[k] = REAL(100.f) * a * (REAL(1) - a / lxr);
+    }
+    sched_kind = scheme ? 1 : 0;
+    row_tag = scheme ? "Temperature " : " Temperature ";
+#elif SW1D_CLI_EQ == 1
+    // KS1D_SweptShared.cu:45, 880-891, 945.
+    prob.equation = SW1D_KS;
+    const REAL dx = (REAL)0.5;
+    const double lx = dv * dx;
+    cout << "KS --- #Blocks: " << bks << " | Length: " << lx << " | Precision: " << prec_name()
+         << " | dt/dx: " << dt / dx << endl;
+    if ((dv & ((tpb - 1) != 0)) || (tpb & 31) != 0) invalid_exit();  // KS:897 as evaluated
+    IC.resize((size_t)dv);
+    for (int k = 0; k < dv; k++) {
+        const REAL x = (REAL)k * (REAL)dx;
+        IC[k] = (REAL)(2.0 * cos(19.0 * (double)x * M_PI / 128.0));
+    }
+    sched_kind = scheme == 0 ? 2 : (scheme == 1 ? 3 : 4);
+    dt_sched = (double)(REAL)dt;  // the wrappers receive dsc.dt (KS:975, 979)
+    row_tag = " Velocity ";
+#else
+    // Euler1D_SweptShared.cu:69, 1127-1172, 1179.
+    prob.equation = SW1D_EULER;
+    const REAL lxr = (REAL)1.0;
+    const double dx = lxr / ((REAL)dv - REAL(2));
+    const REAL gam = (REAL)1.4;
+    (void)gam;
+    mgam = (REAL)0.4;
+    const REAL dt_dx = (REAL)(dt / dx);
+    cout << "Euler --- #Blocks: " << bks << " | Length: " << lxr << " | Precision: " << prec_name()
+         << " | dt/dx: " << dt_dx << endl;
+    if ((dv & ((tpb - 1) != 0)) || (tpb & 31) != 0) invalid_exit();  // Euler:1161 as evaluated
+    if (dt_dx > .21) {
+        cout << "The value of dt/dx (" << dt_dx << ") is too high.  In general it must be <=.21 for stability." << endl;
+        exit(-1);
+    }
+    const REAL bl[3] = {REAL(1), REAL(0), REAL(1) / mgam};
+    const REAL br[3] = {(REAL)0.125, REAL(0), (REAL)(0.1 / (double)mgam)};
+    IC.resize((size_t)dv * 3);
+    for (int k = 0; k < dv; k++)
+        for (int c = 0; c < 3; c++) IC[3 * k + c] = (k < dv / 2) ? bl[c] : br[c];
+    sched_kind = scheme ? 6 : 5;
+#endif
+    if (scheme > 2 && SW1D_CLI_EQ != 1) {
+        cout << "INVALID NUMERIC INPUT!! " << endl;
+        exit(-1);
+    }
+
+    ofstream fwr;
+    fwr.open(argv[7], ios::trunc);
+    fwr.precision(10);
+#if SW1D_CLI_EQ == 0
+    fwr << lx << " " << dv << " " << ds << " " << endl << "Temperature " << 0 << " ";
+    for (int k = 0; k < dv; k++) fwr << IC[k] << " ";
+    fwr << endl;
+#elif SW1D_CLI_EQ == 1
+    fwr << lx << " " << dv << " " << dx << " " << endl << " Velocity " << 0 << " ";
+    for (int k = 0; k < dv; k++) fwr << IC[k] << " ";
+    fwr << endl;
+#else
+    fwr << lxr << " " << (dv - 2) << " " << dx << " " << endl;
+    euler_rows(fwr, 0, IC, dv, mgam, true);
+#endif
+
+#if SW1D_CLI_EQ == 1
+    if (scheme > 2) {
+        // Register-vs-shared self test (KS:682-750): both swept engines from the
+        // same initial condition over one register cycle, compared with tol dt^3.
+        cout << endl << "--- TEST IT ----" << endl << endl << endl;
+        sw1d_problem a = prob, b = prob;
+        a.algorithm = SW1D_SWEPT;
+        b.algorithm = SW1D_ALT;
+        sw1d_plan *pa = nullptr, *pb = nullptr;
+        if (sw1d_plan_create(&pa, &a) || sw1d_plan_create(&pb, &b)) die_lib("plan");
+        vector<REAL> ra(IC), rb(IC);
+        if (sw1d_set_state(pa, IC.data()) || sw1d_set_state(pb, IC.data()) || sw1d_advance(pa, 8) ||
+            sw1d_advance(pb, 8) || sw1d_get_state(pa, ra.data()) || sw1d_get_state(pb, rb.data()))
+            die_lib("self test");
+        const REAL tol = (REAL)(dt * dt * dt);
+        cout << "Tolerance: " << tol << endl;
+        cout << "---- TEST DOWN --------" << endl;
+        int t = 0;
+        for (int k = 0; k < dv; k++) t += (fabs(ra[k] - rb[k]) > tol);
+        if (!t) {
+            cout << "EQUAL!" << endl;
+        } else {
+            cout << "NOT EQUAL!" << endl;
+            cout << t << " out of " << dv << endl;
+        }
+        sw1d_plan_destroy(pa);
+        sw1d_plan_destroy(pb);
+        exit(-1);
+    }
+#endif
+
+    const Rows rows = schedule(sched_kind, dt, atof(argv[4]), freq, tpb, dt_sched);
+
+    const auto t0 = chrono::steady_clock::now();
+    sw1d_plan* plan = nullptr;
+    if (sw1d_plan_create(&plan, &prob) != SW1D_OK) {
+        printf("CUDA error: %s\n", sw1d_last_error());
+        exit(-1);
+    }
+    if (sw1d_set_state(plan, IC.data()) != SW1D_OK) die_lib("set_state");
+
+#if SW1D_CLI_EQ == 2
+    cout << scheme << " ";
+#endif
+    if (scheme == 0)
+        cout << "Classic scheme" << endl;
+    else if (scheme == 2 && SW1D_CLI_EQ == 1)
+        cout << "GPU Register scheme" << endl;
+    else if (scheme == 2)
+        cout << "Hybrid Swept scheme" << endl;
+    else
+        cout << "GPU only Swept scheme" << endl;
+
+    vector<REAL> Tf(IC.size());
+    int64_t done = 0;
+    for (size_t i = 0; i + 1 < rows.steps.size(); ++i) {
+        if (sw1d_advance(plan, rows.steps[i] - done) != SW1D_OK) die_lib("advance");
+        done = rows.steps[i];
+        if (sw1d_get_state(plan, Tf.data()) != SW1D_OK) die_lib("get_state");
+        const double t = rows.times[i];
+#if SW1D_CLI_EQ == 2
+        euler_rows(fwr, t, Tf, dv, mgam, false);
+#else
+        fwr << row_tag << t << " ";
+        for (int k = 0; k < dv; k++) fwr << Tf[k] << " ";
+        fwr << endl;
+#endif
+    }
+    const double tfm = rows.times.back();
+    if (sw1d_advance(plan, rows.steps.back() - done) != SW1D_OK) die_lib("advance");
+    if (sw1d_get_state(plan, Tf.data()) != SW1D_OK) die_lib("get_state");
+#if SW1D_CLI_EQ == 0
+    {
+        const double t_full = scheme ? dt * (double)tpb : dt + dt;
+        cout << tfm << " " << tf << " " << t_full << endl;  // Heat:512, 684
+    }
+#endif
+    sw1d_plan_destroy(plan);
+    const auto t1 = chrono::steady_clock::now();
+
+    const double timed = chrono::duration<double, micro>(t1 - t0).count();
+#if SW1D_CLI_EQ == 1
+    const int n_timesteps = tfm / dt;  // KS:997
+    const double per_ts = timed / (double)n_timesteps;
+#else
+    const double n_timesteps = tfm / dt;
+    const double per_ts = timed / n_timesteps;
+#endif
+    cout << n_timesteps << " timesteps" << endl;
+    cout << "Averaged " << per_ts << " microseconds (us) per timestep" << endl;
+
+    if (argc > 8) {
+        ofstream ftime;
+        ftime.open(argv[8], ios::app);
+        ftime << dv << "\t" << tpb << "\t" << per_ts << endl;
+        ftime.close();
+    }
+
+#if SW1D_CLI_EQ == 0
+    fwr << "Temperature " << tfm << " ";
+    for (int k = 0; k < dv; k++) fwr << Tf[k] << " ";
+#elif SW1D_CLI_EQ == 1
+    fwr << " Velocity " << tfm << " ";
+    for (int k = 0; k < dv; k++) fwr << Tf[k] << " ";
+    fwr << endl;
+#else
+    euler_rows(fwr, tfm, Tf, dv, mgam, false);
+#endif
+    fwr.close();
+    return 0;
+}

@@ -1,0 +1,61 @@
+// Internal interface between the host plan code (sw1d_plan.cpp) and the
+// kernels (sw1d_kernels.hip).  Not part of the public C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "sw1d_stencil.hpp"
+
+namespace sw1d {
+
+enum Eq : int { EQ_HEAT = 0, EQ_KS = 1, EQ_EULER = 2 };
+enum Bc : int { BC_MIRROR = 1, BC_PERIODIC = 2, BC_DIRICHLET = 3 };
+
+// Swept tile kinds.  UP: up-triangle only (Heat:147-196, KS:209-263,
+// Euler:437-484).  DOWN: down-triangle only (Heat:198-244, KS:265-308,
+// Euler:495-536).  DIAMOND: down then up (wholeDiamond / splitDiamond).
+enum Tile : int { TILE_UP = 0, TILE_DOWN = 1, TILE_DIAMOND = 2 };
+
+// Number of values each point carries in the staggered-level state:
+// heat (even row, odd row); KS (u, u_half); Euler (rho, m, E, rho_half,
+// m_half, E_half).
+inline int eq_components(int eq) { return eq == EQ_EULER ? 6 : 2; }
+// Stencil radius of one stage (heat: one FTCS step; KS/Euler: one RK2 stage).
+inline int eq_radius(int eq) { return eq == EQ_HEAT ? 1 : 2; }
+// RK stages per time step.
+inline int eq_stages(int eq) { return eq == EQ_HEAT ? 1 : 2; }
+
+// One swept launch over `nnodes` nodes of W = 64*P points.  Node k covers the
+// points (k*W + offset + j) mod n, j = 0..W-1.  `in` and `out` point at
+// component-major arrays of nc*n values.
+struct SweptLaunch {
+    int eq, bc, P, tile;
+    int64_t n, nnodes, node_first, offset;
+    const void* in;
+    void* out;
+};
+
+struct Coefs {
+    // heat
+    double fo_d, fo2_d; float fo_f, fo2_f;
+    KsCoef<double> ks_d; KsCoef<float> ks_f;
+    EulerCoef<double> eu_d; EulerCoef<float> eu_f;
+};
+
+// Launchers (sw1d_kernels.hip).  All return a hipError_t.
+hipError_t launch_swept(const SweptLaunch& L, bool f64, const Coefs& c, hipStream_t s);
+// One classic stage: heat (stage ignored), KS / Euler stage 0 = predictor,
+// 1 = corrector.  State layout as above; heat ping-pongs between in and out,
+// KS / Euler keep u and u_half in the same buffer.
+hipError_t launch_classic(int eq, int bc, bool f64, int stage, int64_t n, const void* in,
+                          void* out, const Coefs& c, hipStream_t s);
+// Checks the lane-shift primitive the swept kernels rely on; returns 1 if
+// every lane received its neighbour's value, 0 otherwise, <0 on HIP error.
+int check_lane_shifts();
+
+// Largest node width (points) supported per equation and precision.
+inline int max_points_per_lane(int eq, bool f64) { return eq == EQ_EULER ? (f64 ? 8 : 8) : 16; }
+inline int min_points_per_lane(int eq) { return eq == EQ_HEAT ? 1 : 2; }
+
+}  // namespace sw1d

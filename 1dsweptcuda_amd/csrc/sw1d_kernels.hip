@@ -1,0 +1,536 @@
+// gfx950 kernels of the 1D swept-rule solver.
+//
+// Swept tiles (up-triangle, down-triangle, diamond) — one wave64 per node.
+//   A node is W = 64*P consecutive points; lane l owns points l*P .. l*P+P-1
+//   in VGPRs for the whole launch.  The stencil neighbours of a lane's first
+//   and last points come from the adjacent lanes through DPP wave shifts
+//   (v_mov_b32_dpp wave_shr:1 / wave_shl:1), so a whole triangle or diamond
+//   runs with no LDS traffic and no barrier.  Lanes 0 and 63 receive the
+//   values beyond the node (its halo) through the DPP "old" operand.
+//
+//   Each point carries two time levels of its state — heat: the even and odd
+//   level rows; KS: (u, u_half); Euler: the conserved triple and its
+//   predictor — at its own position in the tile, so the data
+//   exchanged between launches (the reference's right/left edge arrays,
+//   Heat:69-119, KS:75-109, Euler:95-174) is simply one staggered-level state
+//   array per launch, read once and written once, coalesced.
+//
+//   Stage s of a tile updates exactly the points inside the triangle or
+//   diamond at that level: down phase |2x-(W-1)| < 2Rs, up phase
+//   |2x-(W-1)| < W - 2Rm (R = stencil radius of one stage).  A point outside
+//   keeps its value, so a tile computes every point-update of the classic
+//   scheme once, with the same operands, and the result is bitwise the
+//   classic one (the reference's own consistency invariant,
+//   ResultPlots/ExactTesting/consistency.out).
+//
+//   Node placement: node k covers (k*W + offset + j) mod n.  offset 0 gives
+//   the node-aligned tiles (upTriangle, wholeDiamond, downTriangle); offset
+//   n - W/2 gives the tiles centred on node boundaries, whose node 0 straddles
+//   the domain ends (splitDiamond, Heat:335-408, Euler:666-746).  Boundary
+//   conditions are applied by global index wherever the boundary falls.
+//
+// Classic kernels — one thread per point, one launch per stage
+//   (classicHeat Heat:132-145, classicKS KS:189-207, classicEuler Euler:400-426).
+
+#include "sw1d_internal.hpp"
+
+namespace sw1d {
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kWavesPerBlock = 4;
+constexpr int kBlock = kWave * kWavesPerBlock;
+constexpr int kClassicBlock = 256;
+
+// ---------------------------------------------------------- lane shifts ---
+// DPP wave_shr:1 (0x138): lane l receives lane l-1; lane 0 keeps `old`.
+// DPP wave_shl:1 (0x130): lane l receives lane l+1; lane 63 keeps `old`.
+__device__ __forceinline__ int shr1_i(int v, int old)
+{
+    return __builtin_amdgcn_update_dpp(old, v, 0x138, 0xf, 0xf, false);
+}
+__device__ __forceinline__ int shl1_i(int v, int old)
+{
+    return __builtin_amdgcn_update_dpp(old, v, 0x130, 0xf, 0xf, false);
+}
+
+__device__ __forceinline__ double from_left(double v, double old)
+{
+    const int lo = shr1_i(__double2loint(v), __double2loint(old));
+    const int hi = shr1_i(__double2hiint(v), __double2hiint(old));
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double from_right(double v, double old)
+{
+    const int lo = shl1_i(__double2loint(v), __double2loint(old));
+    const int hi = shl1_i(__double2hiint(v), __double2hiint(old));
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float from_left(float v, float old)
+{
+    return __int_as_float(shr1_i(__float_as_int(v), __float_as_int(old)));
+}
+__device__ __forceinline__ float from_right(float v, float old)
+{
+    return __int_as_float(shl1_i(__float_as_int(v), __float_as_int(old)));
+}
+
+// e[0 .. P+2R-1] = values at node-local x = l*P - R .. l*P + P + R - 1.
+template <typename T, int P, int R>
+__device__ __forceinline__ void gather(const T (&v)[P], const T (&hl)[R], const T (&hr)[R],
+                                       T (&e)[P + 2 * R])
+{
+    static_assert(P >= R, "a lane must hold at least one stencil radius");
+#pragma unroll
+    for (int p = 0; p < P; ++p) e[R + p] = v[p];
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        e[R - 1 - i] = from_left(v[P - 1 - i], hl[i]);
+        e[R + P + i] = from_right(v[i], hr[i]);
+    }
+}
+
+template <typename T, int P>
+__device__ __forceinline__ void load_pts(const T* __restrict__ a, int64_t i0, T (&v)[P])
+{
+#pragma unroll
+    for (int p = 0; p < P; ++p) v[p] = a[i0 + p];
+}
+template <typename T, int P>
+__device__ __forceinline__ void store_pts(T* __restrict__ a, int64_t i0, const T (&v)[P])
+{
+#pragma unroll
+    for (int p = 0; p < P; ++p) a[i0 + p] = v[p];
+}
+
+struct NodeGeom {
+    int64_t base;  // global index of node-local x = 0
+    int64_t lb;    // global index of this lane's first point
+    int lane;
+};
+
+template <int P>
+__device__ __forceinline__ bool node_geom(int64_t n, int64_t nnodes, int64_t node_first,
+                                          int64_t offset, NodeGeom& g)
+{
+    constexpr int W = kWave * P;
+    g.lane = threadIdx.x & (kWave - 1);
+    const int64_t node = node_first + (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    if (node >= nnodes) return false;
+    int64_t b = node * W + offset;
+    b = b >= n ? b - n : b;
+    g.base = b;
+    int64_t lb = b + (int64_t)g.lane * P;
+    g.lb = lb >= n ? lb - n : lb;
+    return true;
+}
+
+__device__ __forceinline__ int64_t wrap(int64_t i, int64_t n)
+{
+    return i < 0 ? i + n : (i >= n ? i - n : i);
+}
+
+// ----------------------------------------------------------- heat tiles ---
+// Two values per point, the even-level and the odd-level row (the two rows of
+// the reference's working array, Heat:151-164): a point just inside a
+// triangle edge is one level ahead of its outer neighbour, and the neighbour
+// needs the level before.  Odd stages write row 1 from row 0, even stages
+// row 0 from row 1.
+template <typename T, int P, int TILE, bool MIRROR>
+__global__ __launch_bounds__(kBlock) void k_heat_swept(const T* __restrict__ in,
+                                                       T* __restrict__ out, int64_t n,
+                                                       int64_t nnodes, int64_t node_first,
+                                                       int64_t offset, T fo, T fo2)
+{
+    constexpr int W = kWave * P, H = W / 2;
+    NodeGeom g;
+    if (!node_geom<P>(n, nnodes, node_first, offset, g)) return;
+
+    const T* __restrict__ in1 = in + n;
+    T r0[P], r1[P];
+    load_pts<T, P>(in, g.lb, r0);
+    load_pts<T, P>(in1, g.lb, r1);
+    T hl0[1] = {in[wrap(g.base - 1, n)]}, hr0[1] = {in[wrap(g.base + W, n)]};
+    T hl1[1] = {in1[wrap(g.base - 1, n)]}, hr1[1] = {in1[wrap(g.base + W, n)]};
+    int d[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) d[p] = abs(2 * (g.lane * P + p) - (W - 1));
+    // Insulated ends (Heat:140-142): the left neighbour of point 0 is point 1,
+    // the right neighbour of point n-1 is point n-2.
+    const bool g0 = MIRROR && g.lb == 0;
+    const bool gN = MIRROR && g.lb + P - 1 == n - 1;
+
+    auto stage = [&](int thr, const T (&src)[P], const T (&hl)[1], const T (&hr)[1], T (&dst)[P]) {
+        T e[P + 2];
+        gather<T, P, 1>(src, hl, hr, e);
+        if (MIRROR) {
+            e[0] = g0 ? e[2] : e[0];
+            e[P + 1] = gN ? e[P - 1] : e[P + 1];
+        }
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const T nv = heat_ftcs(e[p], e[p + 1], e[p + 2], fo, fo2);
+            dst[p] = (d[p] <= thr) ? nv : dst[p];
+        }
+    };
+    if (TILE != TILE_UP)
+        for (int s = 1; s <= H; s += 2) {
+            stage(2 * s - 1, r0, hl0, hr0, r1);
+            stage(2 * s + 1, r1, hl1, hr1, r0);
+        }
+    if (TILE != TILE_DOWN) {
+        for (int m = 1; m < H - 1; m += 2) {
+            stage(W - 1 - 2 * m, r0, hl0, hr0, r1);
+            stage(W - 3 - 2 * m, r1, hl1, hr1, r0);
+        }
+        stage(W - 1 - 2 * (H - 1), r0, hl0, hr0, r1);
+    }
+    store_pts<T, P>(out, g.lb, r0);
+    store_pts<T, P>(out + n, g.lb, r1);
+}
+
+// ------------------------------------------------------------- KS tiles ---
+template <typename T, int P, int TILE>
+__global__ __launch_bounds__(kBlock) void k_ks_swept(const T* __restrict__ in,
+                                                     T* __restrict__ out, int64_t n,
+                                                     int64_t nnodes, int64_t node_first,
+                                                     int64_t offset, KsCoef<T> k)
+{
+    constexpr int W = kWave * P, H = W / 4;
+    NodeGeom g;
+    if (!node_geom<P>(n, nnodes, node_first, offset, g)) return;
+
+    const T* __restrict__ in_h = in + n;
+    T u[P], uh[P];
+    load_pts<T, P>(in, g.lb, u);
+    load_pts<T, P>(in_h, g.lb, uh);
+    T hlu[2], hru[2], hlh[2], hrh[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int64_t il = wrap(g.base - 1 - i, n), ir = wrap(g.base + W + i, n);
+        hlu[i] = in[il]; hru[i] = in[ir];
+        hlh[i] = in_h[il]; hrh[i] = in_h[ir];
+    }
+    int d[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) d[p] = abs(2 * (g.lane * P + p) - (W - 1));
+
+    // Predictor stage (KS:159-164): u_half = u - dt/2 F(u).
+    auto stage_a = [&](int thr) {
+        T e[P + 4];
+        gather<T, P, 2>(u, hlu, hru, e);
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const T nv = ks_predict(e[p], e[p + 1], e[p + 2], e[p + 3], e[p + 4], k);
+            uh[p] = (d[p] <= thr) ? nv : uh[p];
+        }
+    };
+    // Corrector stage (KS:166-171, :202): u += -dt F(u_half).
+    auto stage_b = [&](int thr) {
+        T e[P + 4];
+        gather<T, P, 2>(uh, hlh, hrh, e);
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const T nv = ks_correct(u[p], e[p], e[p + 1], e[p + 2], e[p + 3], e[p + 4], k);
+            u[p] = (d[p] <= thr) ? nv : u[p];
+        }
+    };
+    if (TILE != TILE_UP)
+        for (int s = 1; s <= H; s += 2) { stage_a(4 * s - 1); stage_b(4 * s + 3); }
+    if (TILE != TILE_DOWN) {
+        for (int m = 1; m < H - 1; m += 2) { stage_a(W - 1 - 4 * m); stage_b(W - 5 - 4 * m); }
+        stage_a(W - 1 - 4 * (H - 1));
+    }
+    store_pts<T, P>(out, g.lb, u);
+    store_pts<T, P>(out + n, g.lb, uh);
+}
+
+// ---------------------------------------------------------- Euler tiles ---
+template <typename T, int P, int TILE>
+__global__ __launch_bounds__(kBlock) void k_euler_swept(const T* __restrict__ in,
+                                                        T* __restrict__ out, int64_t n,
+                                                        int64_t nnodes, int64_t node_first,
+                                                        int64_t offset, EulerCoef<T> k)
+{
+    constexpr int W = kWave * P, H = W / 4;
+    NodeGeom g;
+    if (!node_geom<P>(n, nnodes, node_first, offset, g)) return;
+
+    // Components: 0..2 conserved (rho, rho*u, E), 3..5 their predictor.
+    T q[6][P];
+    T hl[6][2], hr[6][2];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        load_pts<T, P>(in + c * n, g.lb, q[c]);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            hl[c][i] = in[c * n + wrap(g.base - 1 - i, n)];
+            hr[c][i] = in[c * n + wrap(g.base + W + i, n)];
+        }
+    }
+    int d[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) d[p] = abs(2 * (g.lane * P + p) - (W - 1));
+    // Ghost cells 0 and n-1 hold the Dirichlet states; cells 1 and n-2 drop
+    // the outer pressure difference (Euler:405-425).
+    const bool g0 = g.lb == 0;
+    const bool gN = g.lb + P - 1 == n - 1;
+
+    // src = 0: predictor from the conserved values (Euler:307-350);
+    // src = 3: corrector from the predictor values (Euler:353-389, :420).
+    auto stage = [&](int thr, auto src_tag) {
+        constexpr int src = decltype(src_tag)::value;
+        constexpr int dst = 3 - src;
+        T ex[P + 4], ey[P + 4], ez[P + 4];
+        gather<T, P, 2>(q[src + 0], hl[src + 0], hr[src + 0], ex);
+        gather<T, P, 2>(q[src + 1], hl[src + 1], hr[src + 1], ey);
+        gather<T, P, 2>(q[src + 2], hl[src + 2], hr[src + 2], ez);
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const Q3<T> s0{ex[p], ey[p], ez[p]}, s1{ex[p + 1], ey[p + 1], ez[p + 1]};
+            const Q3<T> s2{ex[p + 2], ey[p + 2], ez[p + 2]}, s3{ex[p + 3], ey[p + 3], ez[p + 3]};
+            const Q3<T> s4{ex[p + 4], ey[p + 4], ez[p + 4]};
+            const bool fl = (p == 1) ? g0 : false;
+            const bool fr = (p == P - 2) ? gN : false;
+            Q3<T> nv;
+            if (src == 0) {
+                nv = eu_predict(s0, s1, s2, s3, s4, fl, fr, k);
+            } else {
+                const Q3<T> uc{q[0][p], q[1][p], q[2][p]};
+                nv = eu_correct(uc, s0, s1, s2, s3, s4, fl, fr, k);
+            }
+            if (p == 0 && g0) nv = k.bl;
+            if (p == P - 1 && gN) nv = k.br;
+            const bool act = d[p] <= thr;
+            q[dst + 0][p] = act ? nv.x : q[dst + 0][p];
+            q[dst + 1][p] = act ? nv.y : q[dst + 1][p];
+            q[dst + 2][p] = act ? nv.z : q[dst + 2][p];
+        }
+    };
+    using A = std::integral_constant<int, 0>;
+    using B = std::integral_constant<int, 3>;
+    if (TILE != TILE_UP)
+        for (int s = 1; s <= H; s += 2) { stage(4 * s - 1, A{}); stage(4 * s + 3, B{}); }
+    if (TILE != TILE_DOWN) {
+        for (int m = 1; m < H - 1; m += 2) { stage(W - 1 - 4 * m, A{}); stage(W - 5 - 4 * m, B{}); }
+        stage(W - 1 - 4 * (H - 1), A{});
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) store_pts<T, P>(out + c * n, g.lb, q[c]);
+}
+
+// ------------------------------------------------------ classic kernels ---
+template <typename T, bool PERIODIC>
+__global__ __launch_bounds__(kClassicBlock) void k_heat_classic(const T* __restrict__ in,
+                                                                T* __restrict__ out, int64_t n,
+                                                                T fo, T fo2)
+{
+    const int64_t i = (int64_t)blockIdx.x * kClassicBlock + threadIdx.x;
+    if (i >= n) return;
+    int64_t l, r;
+    if (PERIODIC) {
+        l = i == 0 ? n - 1 : i - 1;
+        r = i == n - 1 ? 0 : i + 1;
+    } else {
+        l = i == 0 ? 1 : i - 1;
+        r = i == n - 1 ? n - 2 : i + 1;
+    }
+    out[i] = heat_ftcs(in[l], in[i], in[r], fo, fo2);
+}
+
+template <typename T, int STAGE>
+__global__ __launch_bounds__(kClassicBlock) void k_ks_classic(T* __restrict__ buf, int64_t n,
+                                                              KsCoef<T> k)
+{
+    const int64_t i = (int64_t)blockIdx.x * kClassicBlock + threadIdx.x;
+    if (i >= n) return;
+    const T* __restrict__ src = STAGE == 0 ? buf : buf + n;
+    const T ll = src[wrap(i - 2, n)], l = src[wrap(i - 1, n)], c = src[i];
+    const T r = src[wrap(i + 1, n)], rr = src[wrap(i + 2, n)];
+    if (STAGE == 0)
+        buf[n + i] = ks_predict(ll, l, c, r, rr, k);
+    else
+        buf[i] = ks_correct(buf[i], ll, l, c, r, rr, k);
+}
+
+template <typename T, int STAGE>
+__global__ __launch_bounds__(kClassicBlock) void k_euler_classic(T* __restrict__ buf, int64_t n,
+                                                                 EulerCoef<T> k)
+{
+    const int64_t i = (int64_t)blockIdx.x * kClassicBlock + threadIdx.x;
+    if (i >= n) return;
+    constexpr int src = STAGE == 0 ? 0 : 3, dst = 3 - src;
+    Q3<T> nv;
+    if (i == 0) {
+        nv = k.bl;
+    } else if (i == n - 1) {
+        nv = k.br;
+    } else {
+        Q3<T> s[5];
+#pragma unroll
+        for (int j = 0; j < 5; ++j) {
+            int64_t idx = i - 2 + j;
+            idx = idx < 0 ? 0 : (idx > n - 1 ? n - 1 : idx);
+            s[j] = Q3<T>{buf[(src + 0) * n + idx], buf[(src + 1) * n + idx], buf[(src + 2) * n + idx]};
+        }
+        const bool fl = i == 1, fr = i == n - 2;
+        if (STAGE == 0) {
+            nv = eu_predict(s[0], s[1], s[2], s[3], s[4], fl, fr, k);
+        } else {
+            const Q3<T> u{buf[i], buf[n + i], buf[2 * n + i]};
+            nv = eu_correct(u, s[0], s[1], s[2], s[3], s[4], fl, fr, k);
+        }
+    }
+    buf[(dst + 0) * n + i] = nv.x;
+    buf[(dst + 1) * n + i] = nv.y;
+    buf[(dst + 2) * n + i] = nv.z;
+}
+
+// ------------------------------------------------------------ lane check ---
+__global__ void k_lane_check(int* ok)
+{
+    const int lane = threadIdx.x;
+    const double v = 1000.0 + lane;
+    const double l = from_left(v, -1.0);
+    const double r = from_right(v, -2.0);
+    const float lf = from_left((float)v, -1.0f);
+    const float rf = from_right((float)v, -2.0f);
+    const double el = lane == 0 ? -1.0 : v - 1.0;
+    const double er = lane == 63 ? -2.0 : v + 1.0;
+    const bool good = (l == el) && (r == er) && (lf == (float)el) && (rf == (float)er);
+    if (!good) atomicAnd(ok, 0);
+}
+
+// ------------------------------------------------------------- dispatch ---
+template <typename T, int P>
+hipError_t swept_eq(const SweptLaunch& L, const Coefs& c, hipStream_t s);
+
+template <typename T> T fo_of(const Coefs& c);
+template <> double fo_of<double>(const Coefs& c) { return c.fo_d; }
+template <> float fo_of<float>(const Coefs& c) { return c.fo_f; }
+template <typename T> T fo2_of(const Coefs& c);
+template <> double fo2_of<double>(const Coefs& c) { return c.fo2_d; }
+template <> float fo2_of<float>(const Coefs& c) { return c.fo2_f; }
+template <typename T> const KsCoef<T>& ks_of(const Coefs& c);
+template <> const KsCoef<double>& ks_of<double>(const Coefs& c) { return c.ks_d; }
+template <> const KsCoef<float>& ks_of<float>(const Coefs& c) { return c.ks_f; }
+template <typename T> const EulerCoef<T>& eu_of(const Coefs& c);
+template <> const EulerCoef<double>& eu_of<double>(const Coefs& c) { return c.eu_d; }
+template <> const EulerCoef<float>& eu_of<float>(const Coefs& c) { return c.eu_f; }
+
+template <typename T, int P, int TILE>
+hipError_t swept_tile(const SweptLaunch& L, const Coefs& c, hipStream_t s)
+{
+    const int64_t count = L.nnodes - L.node_first;
+    if (count <= 0) return hipSuccess;
+    const dim3 grid((unsigned)((count + kWavesPerBlock - 1) / kWavesPerBlock));
+    const T* in = static_cast<const T*>(L.in);
+    T* out = static_cast<T*>(L.out);
+    if (L.eq == EQ_HEAT) {
+        if (L.bc == BC_PERIODIC)
+            hipLaunchKernelGGL((k_heat_swept<T, P, TILE, false>), grid, dim3(kBlock), 0, s, in, out,
+                               L.n, L.nnodes, L.node_first, L.offset, fo_of<T>(c), fo2_of<T>(c));
+        else
+            hipLaunchKernelGGL((k_heat_swept<T, P, TILE, true>), grid, dim3(kBlock), 0, s, in, out,
+                               L.n, L.nnodes, L.node_first, L.offset, fo_of<T>(c), fo2_of<T>(c));
+    } else if (L.eq == EQ_KS) {
+        if constexpr (P >= 2)
+            hipLaunchKernelGGL((k_ks_swept<T, P, TILE>), grid, dim3(kBlock), 0, s, in, out, L.n,
+                               L.nnodes, L.node_first, L.offset, ks_of<T>(c));
+        else
+            return hipErrorInvalidValue;
+    } else {
+        if constexpr (P >= 2 && P <= 8)
+            hipLaunchKernelGGL((k_euler_swept<T, P, TILE>), grid, dim3(kBlock), 0, s, in, out,
+                               L.n, L.nnodes, L.node_first, L.offset, eu_of<T>(c));
+        else
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <typename T, int P>
+hipError_t swept_eq(const SweptLaunch& L, const Coefs& c, hipStream_t s)
+{
+    switch (L.tile) {
+    case TILE_UP: return swept_tile<T, P, TILE_UP>(L, c, s);
+    case TILE_DOWN: return swept_tile<T, P, TILE_DOWN>(L, c, s);
+    case TILE_DIAMOND: return swept_tile<T, P, TILE_DIAMOND>(L, c, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+template <typename T>
+hipError_t swept_prec(const SweptLaunch& L, const Coefs& c, hipStream_t s)
+{
+    switch (L.P) {
+    case 1: return swept_eq<T, 1>(L, c, s);
+    case 2: return swept_eq<T, 2>(L, c, s);
+    case 4: return swept_eq<T, 4>(L, c, s);
+    case 8: return swept_eq<T, 8>(L, c, s);
+    case 16: return swept_eq<T, 16>(L, c, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+template <typename T>
+hipError_t classic_prec(int eq, int bc, int stage, int64_t n, const void* in, void* out,
+                        const Coefs& c, hipStream_t s)
+{
+    const dim3 grid((unsigned)((n + kClassicBlock - 1) / kClassicBlock));
+    if (eq == EQ_HEAT) {
+        const T* a = static_cast<const T*>(in);
+        T* b = static_cast<T*>(out);
+        if (bc == BC_PERIODIC)
+            hipLaunchKernelGGL((k_heat_classic<T, true>), grid, dim3(kClassicBlock), 0, s, a, b, n,
+                               fo_of<T>(c), fo2_of<T>(c));
+        else
+            hipLaunchKernelGGL((k_heat_classic<T, false>), grid, dim3(kClassicBlock), 0, s, a, b, n,
+                               fo_of<T>(c), fo2_of<T>(c));
+    } else if (eq == EQ_KS) {
+        T* b = static_cast<T*>(out);
+        if (stage == 0)
+            hipLaunchKernelGGL((k_ks_classic<T, 0>), grid, dim3(kClassicBlock), 0, s, b, n, ks_of<T>(c));
+        else
+            hipLaunchKernelGGL((k_ks_classic<T, 1>), grid, dim3(kClassicBlock), 0, s, b, n, ks_of<T>(c));
+    } else {
+        T* b = static_cast<T*>(out);
+        if (stage == 0)
+            hipLaunchKernelGGL((k_euler_classic<T, 0>), grid, dim3(kClassicBlock), 0, s, b, n, eu_of<T>(c));
+        else
+            hipLaunchKernelGGL((k_euler_classic<T, 1>), grid, dim3(kClassicBlock), 0, s, b, n, eu_of<T>(c));
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_swept(const SweptLaunch& L, bool f64, const Coefs& c, hipStream_t s)
+{
+    return f64 ? swept_prec<double>(L, c, s) : swept_prec<float>(L, c, s);
+}
+
+hipError_t launch_classic(int eq, int bc, bool f64, int stage, int64_t n, const void* in,
+                          void* out, const Coefs& c, hipStream_t s)
+{
+    return f64 ? classic_prec<double>(eq, bc, stage, n, in, out, c, s)
+               : classic_prec<float>(eq, bc, stage, n, in, out, c, s);
+}
+
+int check_lane_shifts()
+{
+    int* d = nullptr;
+    if (hipMalloc(&d, sizeof(int)) != hipSuccess) return -1;
+    const int one = 1;
+    int ok = 0;
+    hipError_t e = hipMemcpy(d, &one, sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_lane_check, dim3(1), dim3(64), 0, 0, d);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(&ok, d, sizeof(int), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    return e == hipSuccess ? ok : -1;
+}
+
+}  // namespace sw1d

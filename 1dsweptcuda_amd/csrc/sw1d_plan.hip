@@ -1,0 +1,742 @@
+// Host side of libswept1d: plans, the C ABI, the hybrid CPU node and the
+// schedule entry point.  See include/swept1d.h for the contract.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/swept1d.h"
+#include "sw1d_internal.hpp"
+#include "sw1d_schedule.hpp"
+
+using namespace sw1d;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_OR_FAIL(expr)                                                               \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(SW1D_EDEVICE, "%s failed: %s", #expr, hipGetErrorString(e_));  \
+    } while (0)
+
+enum KernelKind { KK_CLASSIC = 0, KK_UP = 1, KK_DOWN = 2, KK_DIAMOND = 3, KK_SPLIT = 4, KK_COUNT = 5 };
+
+}  // namespace
+
+struct sw1d_plan {
+    sw1d_problem prob{};
+    int eq = 0, bc = 0, alg = 0;
+    bool f64 = true;
+    int64_t n = 0;
+    size_t S = 8;      // sizeof(REAL)
+    int nc = 1;        // values per point
+    int W = 0, P = 0;  // swept node width, points per lane (0: no swept path)
+    int R = 1;         // stage radius
+    int64_t tile_steps = 0;  // steps advanced by one swept tile launch
+    int64_t cycle_steps = 0; // reference t_fullstep / dt
+    Coefs coef{};
+    double consts[9]{};
+    int nconsts = 0;
+
+    void* buf[2]{nullptr, nullptr};  // each nc*n REALs, component-major
+    int cur = 0;                     // buffer holding the current (flat) state
+    hipStream_t stream = nullptr;
+
+    // hybrid resources
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    void* host_win = nullptr;  // pinned (W + 2R) * nc REALs
+
+    // profiling
+    bool profiling = false;
+    struct Pending { int kind; hipEvent_t a, b; };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> event_pool;
+    int64_t k_launches[KK_COUNT]{};
+    double k_ms[KK_COUNT]{};
+};
+
+namespace {
+
+// ------------------------------------------------------------ constants ---
+// Heat1D_SweptShared.cu:56-58, 719-728: fou = th_diff*dt/(ds*ds) in double
+// with REAL th_diff and a REAL ds*ds; Fo and 1-2Fo stored as REAL.
+template <typename T>
+void heat_consts(double dt, double alpha, double ds, T& fo, T& fo2)
+{
+    const T a = (T)alpha, d = (T)ds;
+    const T dd = d * d;
+    const double fou = (double)a * dt / (double)dd;
+    fo = (T)fou;
+    fo2 = (T)(1.0 - 2.0 * fou);
+}
+
+// KS1D_SweptShared.cu:45, 917-926.
+template <typename T>
+KsCoef<T> ks_consts(double dt, double dx_)
+{
+    const T dx = (T)dx_;
+    KsCoef<T> k;
+    k.dx4x = T(1) / (T(4) * dx);
+    k.dx2 = T(1) / (dx * dx);
+    k.dx4 = T(1) / (dx * dx * dx * dx);
+    k.dt = (T)dt;
+    k.dth = (T)(dt * 0.5);
+    return k;
+}
+
+// Euler1D_SweptShared.cu:69, 1127-1135, 1144, 1149.
+template <typename T>
+EulerCoef<T> euler_consts(double dt, int64_t n, double dx_override, double* dx_out)
+{
+    EulerCoef<T> k;
+    k.gam = (T)1.4;
+    k.mgam = (T)0.4;
+    const T lx = (T)1.0;
+    const double dx = dx_override > 0 ? dx_override : (double)(lx / ((T)n - T(2)));
+    *dx_out = dx;
+    k.dtdx = (T)(dt / dx);
+    k.bl = Q3<T>{T(1), T(0), T(1) / k.mgam};
+    k.br = Q3<T>{(T)0.125, T(0), (T)(0.1 / (double)k.mgam)};
+    return k;
+}
+
+void fill_coefs(sw1d_plan* p, double* euler_dx)
+{
+    const sw1d_problem& q = p->prob;
+    Coefs& c = p->coef;
+    const double alpha = q.alpha > 0 ? q.alpha : 8.418e-5;
+    if (p->eq == EQ_HEAT) {
+        const double ds = q.dx > 0 ? q.dx : 0.001;
+        heat_consts<double>(q.dt, alpha, ds, c.fo_d, c.fo2_d);
+        heat_consts<float>(q.dt, alpha, ds, c.fo_f, c.fo2_f);
+        p->nconsts = 2;
+        p->consts[0] = p->f64 ? c.fo_d : c.fo_f;
+        p->consts[1] = p->f64 ? c.fo2_d : c.fo2_f;
+    } else if (p->eq == EQ_KS) {
+        const double dx = q.dx > 0 ? q.dx : 0.5;
+        c.ks_d = ks_consts<double>(q.dt, dx);
+        c.ks_f = ks_consts<float>(q.dt, dx);
+        p->nconsts = 5;
+        if (p->f64) {
+            const KsCoef<double>& k = c.ks_d;
+            const double v[5] = {k.dx4x, k.dx2, k.dx4, k.dt, k.dth};
+            std::copy(v, v + 5, p->consts);
+        } else {
+            const KsCoef<float>& k = c.ks_f;
+            const double v[5] = {k.dx4x, k.dx2, k.dx4, k.dt, k.dth};
+            std::copy(v, v + 5, p->consts);
+        }
+    } else {
+        double dxd, dxf;
+        c.eu_d = euler_consts<double>(q.dt, p->n, q.dx, &dxd);
+        c.eu_f = euler_consts<float>(q.dt, p->n, q.dx, &dxf);
+        *euler_dx = p->f64 ? dxd : dxf;
+        p->nconsts = 9;
+        if (p->f64) {
+            const EulerCoef<double>& k = c.eu_d;
+            const double v[9] = {k.gam, k.mgam, k.dtdx, k.bl.x, k.bl.y, k.bl.z, k.br.x, k.br.y, k.br.z};
+            std::copy(v, v + 9, p->consts);
+        } else {
+            const EulerCoef<float>& k = c.eu_f;
+            const double v[9] = {k.gam, k.mgam, k.dtdx, k.bl.x, k.bl.y, k.bl.z, k.br.x, k.br.y, k.br.z};
+            std::copy(v, v + 9, p->consts);
+        }
+    }
+}
+
+bool is_pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
+
+// ------------------------------------------------------------ profiling ---
+hipEvent_t take_event(sw1d_plan* p)
+{
+    if (!p->event_pool.empty()) {
+        hipEvent_t e = p->event_pool.back();
+        p->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+void drain_profile(sw1d_plan* p)
+{
+    for (auto& q : p->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, q.a, q.b) == hipSuccess) {
+            p->k_launches[q.kind] += 1;
+            p->k_ms[q.kind] += ms;
+        }
+        p->event_pool.push_back(q.a);
+        p->event_pool.push_back(q.b);
+    }
+    p->pending.clear();
+}
+
+struct ProfScope {
+    sw1d_plan* p;
+    int kind;
+    hipEvent_t a = nullptr;
+    ProfScope(sw1d_plan* p_, int k) : p(p_), kind(k)
+    {
+        if (p->profiling) {
+            a = take_event(p);
+            if (a) (void)hipEventRecord(a, p->stream);
+        }
+    }
+    ~ProfScope()
+    {
+        if (a) {
+            hipEvent_t b = take_event(p);
+            if (b) {
+                (void)hipEventRecord(b, p->stream);
+                p->pending.push_back({kind, a, b});
+            } else {
+                p->event_pool.push_back(a);
+            }
+        }
+    }
+};
+
+// --------------------------------------------------------- GPU launches ---
+int classic_step(sw1d_plan* p)
+{
+    if (p->eq == EQ_HEAT) {
+        ProfScope ps(p, KK_CLASSIC);
+        HIP_OR_FAIL(launch_classic(p->eq, p->bc, p->f64, 0, p->n, p->buf[p->cur],
+                                   p->buf[p->cur ^ 1], p->coef, p->stream));
+        p->cur ^= 1;
+    } else {
+        for (int st = 0; st < 2; ++st) {
+            ProfScope ps(p, KK_CLASSIC);
+            HIP_OR_FAIL(launch_classic(p->eq, p->bc, p->f64, st, p->n, p->buf[p->cur],
+                                       p->buf[p->cur], p->coef, p->stream));
+        }
+    }
+    return SW1D_OK;
+}
+
+int swept_launch(sw1d_plan* p, int tile, bool shifted, int64_t node_first)
+{
+    SweptLaunch L;
+    L.eq = p->eq;
+    L.bc = p->bc;
+    L.P = p->P;
+    L.tile = tile;
+    L.n = p->n;
+    L.nnodes = p->n / p->W;
+    L.node_first = node_first;
+    L.offset = shifted ? p->n - p->W / 2 : 0;
+    L.in = p->buf[p->cur];
+    L.out = p->buf[p->cur ^ 1];
+    const int kind = tile == TILE_UP ? KK_UP : tile == TILE_DOWN ? KK_DOWN
+                   : (shifted ? KK_SPLIT : KK_DIAMOND);
+    {
+        ProfScope ps(p, kind);
+        HIP_OR_FAIL(launch_swept(L, p->f64, p->coef, p->stream));
+    }
+    return SW1D_OK;
+}
+
+// ------------------------------------------------ hybrid CPU node (alg 2) ---
+// Node 0 of a boundary-straddling diamond computed on the host while the GPU
+// computes nodes 1..K-1 (the CPU_diamond of Heat:410-468 / Euler:761-835 and
+// its orchestration, Heat:565-611, Euler:953-1003).  The window holds the
+// node's W points plus R halo points on each side, component-major; the same
+// stage formulas and active sets as the GPU tile give the same bits.
+template <typename T>
+void cpu_node_diamond(sw1d_plan* p, T* win)
+{
+    const int W = p->W, R = p->R, WW = W + 2 * R, H = W / (2 * R);
+    const int64_t n = p->n;
+    // node-local x = 0 is global n - W/2: global n-1 at x = W/2-1, global 0 at x = W/2.
+    const int xg0 = W / 2, xgN = W / 2 - 1;
+    std::vector<T> nv((size_t)WW * 3);
+    auto at = [&](int c, int x) -> T& { return win[(size_t)c * WW + (x + R)]; };
+    auto dist = [&](int x) { return std::abs(2 * x - (W - 1)); };
+    (void)n;
+
+    auto stage = [&](int thr, int st) {
+        if (p->eq == EQ_HEAT) {
+            const T fo = (T)(p->f64 ? p->coef.fo_d : p->coef.fo_f);
+            const T fo2 = (T)(p->f64 ? p->coef.fo2_d : p->coef.fo2_f);
+            const int src = st == 0 ? 0 : 1, dst = 1 - src;
+            for (int x = 0; x < W; ++x) {
+                if (dist(x) > thr) continue;
+                T l = at(src, x - 1), c = at(src, x), r = at(src, x + 1);
+                if (p->bc == BC_MIRROR) {
+                    if (x == xg0) l = r;
+                    if (x == xgN) r = l;
+                }
+                nv[x] = heat_ftcs(l, c, r, fo, fo2);
+            }
+            for (int x = 0; x < W; ++x)
+                if (dist(x) <= thr) at(dst, x) = nv[x];
+        } else if (p->eq == EQ_KS) {
+            const KsCoef<T>& k = *reinterpret_cast<const KsCoef<T>*>(
+                p->f64 ? (const void*)&p->coef.ks_d : (const void*)&p->coef.ks_f);
+            const int src = st == 0 ? 0 : 1;
+            for (int x = 0; x < W; ++x) {
+                if (dist(x) > thr) continue;
+                const T a = at(src, x - 2), b = at(src, x - 1), c = at(src, x), d = at(src, x + 1),
+                        e = at(src, x + 2);
+                nv[x] = st == 0 ? ks_predict(a, b, c, d, e, k) : ks_correct(at(0, x), a, b, c, d, e, k);
+            }
+            const int dst = st == 0 ? 1 : 0;
+            for (int x = 0; x < W; ++x)
+                if (dist(x) <= thr) at(dst, x) = nv[x];
+        } else {
+            const EulerCoef<T>& k = *reinterpret_cast<const EulerCoef<T>*>(
+                p->f64 ? (const void*)&p->coef.eu_d : (const void*)&p->coef.eu_f);
+            const int src = st == 0 ? 0 : 3, dst = 3 - src;
+            for (int x = 0; x < W; ++x) {
+                if (dist(x) > thr) continue;
+                Q3<T> s[5];
+                for (int j = 0; j < 5; ++j)
+                    s[j] = Q3<T>{at(src, x - 2 + j), at(src + 1, x - 2 + j), at(src + 2, x - 2 + j)};
+                const bool fl = (x == xg0 + 1), fr = (x == xgN - 1);
+                Q3<T> q;
+                if (x == xg0) {
+                    q = k.bl;
+                } else if (x == xgN) {
+                    q = k.br;
+                } else if (st == 0) {
+                    q = eu_predict(s[0], s[1], s[2], s[3], s[4], fl, fr, k);
+                } else {
+                    const Q3<T> u{at(0, x), at(1, x), at(2, x)};
+                    q = eu_correct(u, s[0], s[1], s[2], s[3], s[4], fl, fr, k);
+                }
+                nv[3 * x + 0] = q.x; nv[3 * x + 1] = q.y; nv[3 * x + 2] = q.z;
+            }
+            for (int x = 0; x < W; ++x)
+                if (dist(x) <= thr) {
+                    at(dst, x) = nv[3 * x + 0];
+                    at(dst + 1, x) = nv[3 * x + 1];
+                    at(dst + 2, x) = nv[3 * x + 2];
+                }
+        }
+    };
+    for (int s = 1; s <= H; s += 2) { stage(2 * R * s - 1, 0); stage(2 * R * (s + 1) - 1, 1); }
+    for (int m = 1; m < H - 1; m += 2) { stage(W - 1 - 2 * R * m, 0); stage(W - 1 - 2 * R * (m + 1), 1); }
+    stage(W - 1 - 2 * R * (H - 1), 0);
+}
+
+int hybrid_split_diamond(sw1d_plan* p)
+{
+    const int W = p->W, R = p->R, WW = W + 2 * R;
+    const int64_t n = p->n;
+    const size_t S = p->S;
+    char* in = static_cast<char*>(p->buf[p->cur]);
+    char* out = static_cast<char*>(p->buf[p->cur ^ 1]);
+    char* win = static_cast<char*>(p->host_win);
+
+    HIP_OR_FAIL(hipEventRecord(p->ev_in, p->stream));
+    // GPU part: nodes 1..K-1 of the boundary-straddling diamond.
+    int rc = swept_launch(p, TILE_DIAMOND, true, 1);
+    if (rc) return rc;
+    // Node 0 window: global [n - W/2 - R, n) then [0, W/2 + R).
+    HIP_OR_FAIL(hipStreamWaitEvent(p->copy_stream, p->ev_in, 0));
+    const int64_t left = W / 2 + R, right = W / 2 + R;
+    for (int c = 0; c < p->nc; ++c) {
+        char* wc = win + (size_t)c * WW * S;
+        const char* ic = in + (size_t)c * n * S;
+        HIP_OR_FAIL(hipMemcpyAsync(wc, ic + (size_t)(n - left) * S, (size_t)left * S,
+                                   hipMemcpyDeviceToHost, p->copy_stream));
+        HIP_OR_FAIL(hipMemcpyAsync(wc + (size_t)left * S, ic, (size_t)right * S,
+                                   hipMemcpyDeviceToHost, p->copy_stream));
+    }
+    HIP_OR_FAIL(hipStreamSynchronize(p->copy_stream));
+    if (p->f64)
+        cpu_node_diamond<double>(p, reinterpret_cast<double*>(win));
+    else
+        cpu_node_diamond<float>(p, reinterpret_cast<float*>(win));
+    // Node 0 result: window [R, R + W/2) -> [n - W/2, n), [R + W/2, R + W) -> [0, W/2).
+    for (int c = 0; c < p->nc; ++c) {
+        const char* wc = win + (size_t)c * WW * S;
+        char* oc = out + (size_t)c * n * S;
+        HIP_OR_FAIL(hipMemcpyAsync(oc + (size_t)(n - W / 2) * S, wc + (size_t)R * S,
+                                   (size_t)(W / 2) * S, hipMemcpyHostToDevice, p->copy_stream));
+        HIP_OR_FAIL(hipMemcpyAsync(oc, wc + (size_t)(R + W / 2) * S, (size_t)(W / 2) * S,
+                                   hipMemcpyHostToDevice, p->copy_stream));
+    }
+    HIP_OR_FAIL(hipEventRecord(p->ev_out, p->copy_stream));
+    HIP_OR_FAIL(hipStreamWaitEvent(p->stream, p->ev_out, 0));
+    // The pinned window is reused by the next CPU node; its H2D copies must
+    // be complete before the host writes it again.
+    HIP_OR_FAIL(hipEventSynchronize(p->ev_out));
+    return SW1D_OK;
+}
+
+int swept_advance(sw1d_plan* p, int64_t nsteps)
+{
+    const int64_t H = p->tile_steps;
+    int64_t rem = nsteps;
+    if (nsteps >= H) {
+        const int64_t tiles = nsteps / H;  // up + (tiles-1) diamonds + down
+        rem = nsteps - tiles * H;
+        bool shifted = false;
+        int rc = swept_launch(p, TILE_UP, shifted, 0);
+        if (rc) return rc;
+        p->cur ^= 1;
+        for (int64_t i = 0; i + 1 < tiles; ++i) {
+            shifted = !shifted;
+            if (shifted && p->alg == SW1D_ALT && p->eq != EQ_KS)
+                rc = hybrid_split_diamond(p);
+            else
+                rc = swept_launch(p, TILE_DIAMOND, shifted, 0);
+            if (rc) return rc;
+            p->cur ^= 1;
+        }
+        shifted = !shifted;
+        rc = swept_launch(p, TILE_DOWN, shifted, 0);
+        if (rc) return rc;
+        p->cur ^= 1;
+    }
+    for (int64_t s = 0; s < rem; ++s) {
+        int rc = classic_step(p);
+        if (rc) return rc;
+    }
+    return SW1D_OK;
+}
+
+void release(sw1d_plan* p)
+{
+    if (!p) return;
+    if (p->stream) (void)hipStreamSynchronize(p->stream);
+    for (auto& q : p->pending) {
+        (void)hipEventDestroy(q.a);
+        (void)hipEventDestroy(q.b);
+    }
+    for (auto e : p->event_pool) (void)hipEventDestroy(e);
+    for (int i = 0; i < 2; ++i)
+        if (p->buf[i]) (void)hipFree(p->buf[i]);
+    if (p->host_win) (void)hipHostFree(p->host_win);
+    if (p->ev_in) (void)hipEventDestroy(p->ev_in);
+    if (p->ev_out) (void)hipEventDestroy(p->ev_out);
+    if (p->copy_stream) (void)hipStreamDestroy(p->copy_stream);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+}
+
+int g_lane_check = 0;  // 0 unknown, 1 pass, -1 fail
+
+}  // namespace
+
+// ================================================================= C ABI ===
+extern "C" {
+
+int sw1d_abi_version(void) { return SW1D_ABI_VERSION; }
+
+const char* sw1d_last_error(void) { return g_err.c_str(); }
+
+int sw1d_device_check(void)
+{
+    const int r = check_lane_shifts();
+    if (r < 0) return fail(SW1D_EDEVICE, "lane-shift check could not run on the device");
+    g_lane_check = r == 1 ? 1 : -1;
+    return r;
+}
+
+int sw1d_plan_create(sw1d_plan** out, const sw1d_problem* q)
+{
+    if (!out || !q) return fail(SW1D_EINVAL, "null argument");
+    *out = nullptr;
+    if (q->equation < SW1D_HEAT || q->equation > SW1D_EULER)
+        return fail(SW1D_EINVAL, "unknown equation %d", q->equation);
+    if (q->precision != SW1D_F64 && q->precision != SW1D_F32)
+        return fail(SW1D_EINVAL, "unknown precision %d", q->precision);
+    if (q->algorithm < SW1D_CLASSIC || q->algorithm > SW1D_ALT)
+        return fail(SW1D_EINVAL, "unknown algorithm %d", q->algorithm);
+    if (q->nx < 8) return fail(SW1D_EINVAL, "nx must be >= 8 (got %lld)", (long long)q->nx);
+    if (!(q->dt > 0) || !std::isfinite(q->dt)) return fail(SW1D_EINVAL, "dt must be positive");
+
+    sw1d_plan* p = new sw1d_plan();
+    p->prob = *q;
+    p->eq = q->equation;
+    p->alg = q->algorithm;
+    p->f64 = q->precision == SW1D_F64;
+    p->n = q->nx;
+    p->S = p->f64 ? 8 : 4;
+    p->nc = eq_components(p->eq);
+    p->R = eq_radius(p->eq);
+    int bc = q->bc;
+    if (bc == SW1D_BC_DEFAULT) bc = p->eq == EQ_HEAT ? BC_MIRROR : (p->eq == EQ_KS ? BC_PERIODIC : BC_DIRICHLET);
+    if ((p->eq == EQ_HEAT && bc != BC_MIRROR && bc != BC_PERIODIC) ||
+        (p->eq == EQ_KS && bc != BC_PERIODIC) || (p->eq == EQ_EULER && bc != BC_DIRICHLET)) {
+        delete p;
+        return fail(SW1D_EINVAL, "boundary condition %d is not defined for this equation", q->bc);
+    }
+    p->bc = bc;
+
+    double euler_dx = 0;
+    fill_coefs(p, &euler_dx);
+    if (p->eq == EQ_EULER && p->consts[2] > 0.21) {
+        delete p;
+        return fail(SW1D_EUNSTABLE,
+                    "The value of dt/dx (%g) is too high.  In general it must be <=.21 for stability.",
+                    p->consts[2]);
+    }
+
+    // Reference cycle: t_fullstep / dt (Heat:483/541, KS:616/676/755, Euler:863/924).
+    const int tpb = q->tpb > 0 ? q->tpb : 64;
+    if (p->alg == SW1D_CLASSIC) p->cycle_steps = p->eq == EQ_HEAT ? 2 : 1;
+    else if (p->eq == EQ_HEAT) p->cycle_steps = tpb;
+    else if (p->eq == EQ_KS && p->alg == SW1D_ALT) p->cycle_steps = 8;
+    else p->cycle_steps = tpb / 4;
+
+    if (p->alg != SW1D_CLASSIC) {
+        int W = q->node_width > 0 ? q->node_width : tpb;
+        if (p->eq == EQ_KS && p->alg == SW1D_ALT && q->node_width <= 0) W = 128;
+        const int pmin = min_points_per_lane(p->eq), pmax = max_points_per_lane(p->eq, p->f64);
+        if (!is_pow2(W)) {
+            delete p;
+            return fail(SW1D_EINVAL, "swept node width must be a power of two (got %d)", W);
+        }
+        W = std::max(W, 64 * pmin);
+        W = std::min(W, 64 * pmax);
+        while (W > 64 * pmin && (p->n % W != 0 || p->n < 2 * (int64_t)W)) W /= 2;
+        if (p->n % W != 0 || p->n < 2 * (int64_t)W) {
+            delete p;
+            return fail(SW1D_ENOTSUP, "nx=%lld cannot be split into swept nodes of >= %d points",
+                        (long long)p->n, 64 * pmin);
+        }
+        p->W = W;
+        p->P = W / 64;
+        p->tile_steps = p->eq == EQ_HEAT ? W / 2 : W / 8;
+        if (g_lane_check == 0) sw1d_device_check();
+        if (g_lane_check != 1) {
+            delete p;
+            return fail(SW1D_EDEVICE, "DPP lane-shift self check failed on this device");
+        }
+    }
+
+    hipError_t e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        release(p);
+        return fail(SW1D_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    const size_t bytes = (size_t)p->nc * (size_t)p->n * p->S;
+    for (int i = 0; i < 2; ++i) {
+        e = hipMalloc(&p->buf[i], bytes);
+        if (e != hipSuccess) {
+            release(p);
+            return fail(SW1D_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+        }
+        e = hipMemsetAsync(p->buf[i], 0, bytes, p->stream);
+        if (e != hipSuccess) {
+            release(p);
+            return fail(SW1D_EDEVICE, "hipMemset: %s", hipGetErrorString(e));
+        }
+    }
+    if (p->alg == SW1D_ALT && p->eq != EQ_KS) {
+        const size_t wb = (size_t)(p->W + 2 * p->R) * p->nc * p->S;
+        if (hipHostMalloc(&p->host_win, wb, hipHostMallocDefault) != hipSuccess ||
+            hipStreamCreateWithFlags(&p->copy_stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&p->ev_in, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&p->ev_out, hipEventDisableTiming) != hipSuccess) {
+            release(p);
+            return fail(SW1D_ENOMEM, "hybrid host resources could not be allocated");
+        }
+    }
+    e = hipStreamSynchronize(p->stream);
+    if (e != hipSuccess) {
+        release(p);
+        return fail(SW1D_EDEVICE, "hipStreamSynchronize: %s", hipGetErrorString(e));
+    }
+    *out = p;
+    return SW1D_OK;
+}
+
+int sw1d_plan_destroy(sw1d_plan* p)
+{
+    release(p);
+    return SW1D_OK;
+}
+
+static int copy_state(sw1d_plan* p, const void* src, void* dst, bool to_device, hipMemcpyKind kind)
+{
+    const size_t S = p->S;
+    const int64_t n = p->n;
+    char* dev = static_cast<char*>(p->buf[p->cur]);
+    if (p->eq != EQ_EULER) {
+        if (to_device) {
+            HIP_OR_FAIL(hipMemcpyAsync(dev, src, n * S, kind, p->stream));
+            // second value (odd row / u_half) starts as a copy; it is written
+            // before it is read
+            HIP_OR_FAIL(hipMemcpyAsync(dev + n * S, dev, n * S, hipMemcpyDeviceToDevice, p->stream));
+        } else {
+            HIP_OR_FAIL(hipMemcpyAsync(dst, dev, n * S, kind, p->stream));
+        }
+    } else {
+        // interleaved (rho, m, E) cells <-> three component arrays
+        for (int c = 0; c < 3; ++c) {
+            if (to_device) {
+                HIP_OR_FAIL(hipMemcpy2DAsync(dev + (size_t)c * n * S, S,
+                                             static_cast<const char*>(src) + c * S, 3 * S, S, n,
+                                             kind, p->stream));
+                HIP_OR_FAIL(hipMemcpyAsync(dev + (size_t)(c + 3) * n * S, dev + (size_t)c * n * S,
+                                           n * S, hipMemcpyDeviceToDevice, p->stream));
+            } else {
+                HIP_OR_FAIL(hipMemcpy2DAsync(static_cast<char*>(dst) + c * S, 3 * S,
+                                             dev + (size_t)c * n * S, S, S, n, kind, p->stream));
+            }
+        }
+    }
+    HIP_OR_FAIL(hipStreamSynchronize(p->stream));
+    return SW1D_OK;
+}
+
+int sw1d_set_state(sw1d_plan* p, const void* host)
+{
+    if (!p || !host) return fail(SW1D_EINVAL, "null argument");
+    return copy_state(p, host, nullptr, true, hipMemcpyHostToDevice);
+}
+
+int sw1d_get_state(sw1d_plan* p, void* host)
+{
+    if (!p || !host) return fail(SW1D_EINVAL, "null argument");
+    return copy_state(p, nullptr, host, false, hipMemcpyDeviceToHost);
+}
+
+int sw1d_set_state_device(sw1d_plan* p, const void* dptr)
+{
+    if (!p || !dptr) return fail(SW1D_EINVAL, "null argument");
+    return copy_state(p, dptr, nullptr, true, hipMemcpyDeviceToDevice);
+}
+
+int sw1d_get_state_device(sw1d_plan* p, void* dptr)
+{
+    if (!p || !dptr) return fail(SW1D_EINVAL, "null argument");
+    return copy_state(p, nullptr, dptr, false, hipMemcpyDeviceToDevice);
+}
+
+int sw1d_advance(sw1d_plan* p, int64_t nsteps)
+{
+    if (!p) return fail(SW1D_EINVAL, "null plan");
+    if (nsteps < 0) return fail(SW1D_EINVAL, "nsteps must be >= 0");
+    int rc;
+    if (p->alg == SW1D_CLASSIC) {
+        rc = SW1D_OK;
+        for (int64_t s = 0; s < nsteps && rc == SW1D_OK; ++s) rc = classic_step(p);
+    } else {
+        rc = swept_advance(p, nsteps);
+    }
+    return rc;
+}
+
+int sw1d_synchronize(sw1d_plan* p)
+{
+    if (!p) return fail(SW1D_EINVAL, "null plan");
+    HIP_OR_FAIL(hipStreamSynchronize(p->stream));
+    HIP_OR_FAIL(hipGetLastError());
+    if (p->profiling) drain_profile(p);
+    return SW1D_OK;
+}
+
+int sw1d_advance_timed(sw1d_plan* p, int64_t nsteps, double* ms)
+{
+    if (!p || !ms) return fail(SW1D_EINVAL, "null argument");
+    hipEvent_t a, b;
+    HIP_OR_FAIL(hipEventCreate(&a));
+    HIP_OR_FAIL(hipEventCreate(&b));
+    HIP_OR_FAIL(hipEventRecord(a, p->stream));
+    int rc = sw1d_advance(p, nsteps);
+    if (rc == SW1D_OK) {
+        HIP_OR_FAIL(hipEventRecord(b, p->stream));
+        HIP_OR_FAIL(hipEventSynchronize(b));
+        float f = 0.f;
+        HIP_OR_FAIL(hipEventElapsedTime(&f, a, b));
+        *ms = f;
+        rc = sw1d_synchronize(p);
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return rc;
+}
+
+int sw1d_plan_info(const sw1d_plan* p, int64_t info[5])
+{
+    if (!p || !info) return fail(SW1D_EINVAL, "null argument");
+    info[0] = p->W;
+    info[1] = p->alg == SW1D_CLASSIC ? 1 : p->tile_steps;
+    info[2] = p->nc;
+    info[3] = (int64_t)(2 * (size_t)p->nc * (size_t)p->n * p->S);
+    info[4] = p->cycle_steps;
+    return SW1D_OK;
+}
+
+int sw1d_plan_constants(const sw1d_plan* p, double* out, int32_t cap)
+{
+    if (!p || !out) return fail(SW1D_EINVAL, "null argument");
+    if (cap < p->nconsts) return fail(SW1D_EINVAL, "need %d slots", p->nconsts);
+    std::copy(p->consts, p->consts + p->nconsts, out);
+    return p->nconsts;
+}
+
+int sw1d_plan_stream(const sw1d_plan* p, void** s)
+{
+    if (!p || !s) return fail(SW1D_EINVAL, "null argument");
+    *s = (void*)p->stream;
+    return SW1D_OK;
+}
+
+int sw1d_set_profiling(sw1d_plan* p, int32_t on)
+{
+    if (!p) return fail(SW1D_EINVAL, "null plan");
+    HIP_OR_FAIL(hipStreamSynchronize(p->stream));
+    drain_profile(p);
+    p->profiling = on != 0;
+    for (int k = 0; k < KK_COUNT; ++k) { p->k_launches[k] = 0; p->k_ms[k] = 0; }
+    return SW1D_OK;
+}
+
+int sw1d_kernel_stats(sw1d_plan* p, int32_t kind, int64_t* launches, double* ms)
+{
+    if (!p || !launches || !ms) return fail(SW1D_EINVAL, "null argument");
+    if (kind < 0 || kind >= KK_COUNT) return fail(SW1D_EINVAL, "unknown kernel kind %d", kind);
+    HIP_OR_FAIL(hipStreamSynchronize(p->stream));
+    drain_profile(p);
+    *launches = p->k_launches[kind];
+    *ms = p->k_ms[kind];
+    return SW1D_OK;
+}
+
+int sw1d_schedule(int32_t kind, double dt, double tf_arg, double freq, int32_t tpb,
+                  double dt_sched, int64_t* steps, double* times, int64_t cap, int64_t* count)
+{
+    if (!count) return fail(SW1D_EINVAL, "null count");
+    std::vector<SchedRow> rows;
+    if (!run_schedule(kind, dt, tf_arg, freq, tpb, dt_sched, rows))
+        return fail(SW1D_EINVAL, "unknown schedule kind %d or tpb %d", kind, tpb);
+    *count = (int64_t)rows.size();
+    if ((int64_t)rows.size() > cap || (!steps && cap > 0) || (!times && cap > 0))
+        return fail(SW1D_EINVAL, "capacity %lld < %zu rows", (long long)cap, rows.size());
+    for (size_t i = 0; i < rows.size(); ++i) {
+        steps[i] = rows[i].steps;
+        times[i] = rows[i].time;
+    }
+    return SW1D_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,159 @@
+// Point-update formulas of the three equations, written once for the GPU
+// kernels and for the host-side hybrid node.  Every function takes its stencil
+// values as arguments (no array indexing) so that the kernels can feed them
+// from registers.  Operand order follows the reference expressions exactly and
+// all code is compiled with -ffp-contract=off: each operator is one IEEE
+// operation, on the GPU and on the host alike, so every path of this library
+// produces the same bits.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cmath>
+
+#define SW1D_HD __host__ __device__ __forceinline__
+
+namespace sw1d {
+
+// ------------------------------------------------------------------ heat ---
+// Heat1D_SweptShared.cu:121-130 (execFunc): Fo*(T[l]+T[r]) + (1-2Fo)*T[c].
+template <typename T>
+SW1D_HD T heat_ftcs(T l, T c, T r, T fo, T fo2)
+{
+    return fo * (l + r) + fo2 * c;
+}
+
+// -------------------------------------------------------------------- KS ---
+// KS1D_SweptShared.cu:136-155 with the default (multiply by reciprocal)
+// constants; c = {1/(4dx), 1/dx^2, 1/dx^4, dt, dt/2} (KS:917-926).
+template <typename T> struct KsCoef { T dx4x, dx2, dx4, dt, dth; };
+
+template <typename T>
+SW1D_HD T ks_rhs(T ll, T l, T c, T r, T rr, const KsCoef<T>& k)
+{
+    const T conv = (r * r - l * l) * k.dx4x;
+    const T sec = (l + r - T(2) * c) * k.dx2;
+    const T fourth = (ll - T(4) * l + T(6) * c - T(4) * r + rr) * k.dx4;
+    return conv + sec + fourth;
+}
+// KS:159-164 (stutterStep): predictor u_half = u - dt/2 * F(u).
+template <typename T>
+SW1D_HD T ks_predict(T ll, T l, T c, T r, T rr, const KsCoef<T>& k)
+{
+    return c - k.dth * ks_rhs(ll, l, c, r, rr, k);
+}
+// KS:166-171 (finalStep) and the "+=" of KS:202: u + (-dt * F(u_half)).
+template <typename T>
+SW1D_HD T ks_correct(T u, T ll, T l, T c, T r, T rr, const KsCoef<T>& k)
+{
+    return u + (-k.dt * ks_rhs(ll, l, c, r, rr, k));
+}
+
+// ----------------------------------------------------------------- Euler ---
+// Conserved state (rho, rho*u, E) of one cell.  Own three-component type;
+// the arithmetic is written out per component.
+template <typename T> struct Q3 { T x, y, z; };
+
+template <typename T> struct EulerCoef { T gam, mgam, dtdx; Q3<T> bl, br; };
+
+// Euler1D_SweptShared.cu:183-193 (pressure).
+template <typename T>
+SW1D_HD T eu_pressure(const Q3<T>& c, T mgam)
+{
+    return mgam * (c.z - (T(0.5) * c.y * c.y / c.x));
+}
+
+// Euler:222-228 (limitor): cur + 1/2*min(ratio,1)*(other-cur).
+template <typename T>
+SW1D_HD Q3<T> eu_limit(const Q3<T>& cur, const Q3<T>& oth, T ratio)
+{
+    const T m = T(0.5) * (ratio < T(1) ? ratio : T(1));
+    return Q3<T>{cur.x + m * (oth.x - cur.x), cur.y + m * (oth.y - cur.y),
+                 cur.z + m * (oth.z - cur.z)};
+}
+
+// Euler:237-257 (eulerFlux): F(L)+F(R) without the factor 1/2, and
+// Euler:266-292 (eulerSpectral): Roe-averaged (sqrt(gam*p/rho)+|u|)*(L-R).
+template <typename T>
+SW1D_HD void eu_flux_diss(const Q3<T>& L, const Q3<T>& R, const EulerCoef<T>& k,
+                          Q3<T>& F, Q3<T>& D)
+{
+    const T uL = L.y / L.x;
+    const T uR = R.y / R.x;
+    const T pL = eu_pressure(L, k.mgam);
+    const T pR = eu_pressure(R, k.mgam);
+    F.x = (L.y + R.y);
+    F.y = (L.y * uL + R.y * uR + pL + pR);
+    F.z = (L.z * uL + R.z * uR + uL * pL + uR * pR);
+
+    const T sl = sqrt(L.x);
+    const T sr = sqrt(R.x);
+    const T hx = sl * sr;
+    const T den = T(1) / (hx * (sl + sr));
+    const T hy = (sl * R.y + sr * L.y) * den;
+    const T hz = (sl * R.z + sr * L.z) * den;
+    const T ph = k.mgam * (hz - T(0.5) * hy * hy);  // Euler:202-212 (pressureHalf)
+    const T s = (sqrt(ph * k.gam) + fabs(hy));
+    D.x = s * (L.x - R.x);
+    D.y = s * (L.y - R.y);
+    D.z = s * (L.z - R.z);
+}
+
+// Division-free pressure difference of Euler:311-322.
+template <typename T>
+SW1D_HD T eu_pdiff(const Q3<T>& a, const Q3<T>& b)
+{
+    return (T(2) * b.x * a.x * (b.z - a.z) + (a.y * a.y * b.x - b.y * b.y * a.x));
+}
+
+// Net flux of cell 2 of the five cells s0..s4 (Euler:307-341 / 353-380).
+// flagL / flagR drop the outermost pressure difference (cells 1 and n-2).
+template <typename T>
+SW1D_HD Q3<T> eu_net_flux(const Q3<T>& s0, const Q3<T>& s1, const Q3<T>& s2,
+                          const Q3<T>& s3, const Q3<T>& s4, bool flagL, bool flagR,
+                          const EulerCoef<T>& k)
+{
+    const T pLL = flagL ? T(0) : eu_pdiff(s0, s1);
+    const T pL = eu_pdiff(s1, s2);
+    const T pR = eu_pdiff(s2, s3);
+    const T pRR = flagR ? T(0) : eu_pdiff(s3, s4);
+    const bool mid = (!pL || !pR || ((pL < 0) != (pR < 0)));
+
+    Q3<T> tL = (!pLL || !pL || ((pLL < 0) != (pL < 0))) ? s1
+             : eu_limit(s1, s2, (s0.x * pL / (s2.x * pLL)));
+    Q3<T> tR = mid ? s2 : eu_limit(s2, s1, (s3.x * pL / (s1.x * pR)));
+    Q3<T> acc, F, D;
+    eu_flux_diss(tL, tR, k, acc, D);  // flux = eulerFlux(...)
+    acc.x = acc.x + D.x; acc.y = acc.y + D.y; acc.z = acc.z + D.z;
+
+    tL = mid ? s2 : eu_limit(s2, s3, (s1.x * pR / (s3.x * pL)));
+    tR = (!pRR || !pR || ((pRR < 0) != (pR < 0))) ? s3
+       : eu_limit(s3, s2, (s4.x * pR / (s2.x * pRR)));
+    eu_flux_diss(tL, tR, k, F, D);
+    acc.x = acc.x - F.x; acc.y = acc.y - F.y; acc.z = acc.z - F.z;
+    acc.x = acc.x - D.x; acc.y = acc.y - D.y; acc.z = acc.z - D.z;
+    return acc;
+}
+
+// Euler:307-350 (eulerStutterStep): Q + (1/4 dt/dx) * flux.
+template <typename T>
+SW1D_HD Q3<T> eu_predict(const Q3<T>& s0, const Q3<T>& s1, const Q3<T>& s2,
+                         const Q3<T>& s3, const Q3<T>& s4, bool fl, bool fr,
+                         const EulerCoef<T>& k)
+{
+    const Q3<T> f = eu_net_flux(s0, s1, s2, s3, s4, fl, fr, k);
+    const T c = T(0.25) * k.dtdx;
+    return Q3<T>{s2.x + c * f.x, s2.y + c * f.y, s2.z + c * f.z};
+}
+
+// Euler:353-389 (eulerFinalStep) plus the "+=" of Euler:420: u + (1/2 dt/dx) flux.
+template <typename T>
+SW1D_HD Q3<T> eu_correct(const Q3<T>& u, const Q3<T>& s0, const Q3<T>& s1,
+                         const Q3<T>& s2, const Q3<T>& s3, const Q3<T>& s4, bool fl,
+                         bool fr, const EulerCoef<T>& k)
+{
+    const Q3<T> f = eu_net_flux(s0, s1, s2, s3, s4, fl, fr, k);
+    const T c = T(0.5) * k.dtdx;
+    return Q3<T>{u.x + c * f.x, u.y + c * f.y, u.z + c * f.z};
+}
+
+}  // namespace sw1d

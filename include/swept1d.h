@@ -1,0 +1,141 @@
+/*
+ * swept1d.h — C ABI of libswept1d, the MI355X (gfx950) 1D swept-rule solver.
+ *
+ * The reference (Niemeyer-Research-Group/1DSweptCUDA) has no library API: its
+ * boundary is the process (argv + .dat/.txt files) of the executables
+ * bin/{Heat,KS,Euler}{Single,Double}Out (SweptSource/Makefile:17-35).  This
+ * header is the C entry surface those executables are rebuilt on; each entry
+ * point names the reference code it replaces.  Plain C types only: pointers,
+ * sizes, integers, doubles.
+ *
+ * Conventions
+ *   - Every function returns 0 (SW1D_OK) on success and a negative SW1D_E*
+ *     code on failure; sw1d_last_error() gives the message (thread-local).
+ *     The library never calls exit().
+ *   - The caller owns every host buffer; a plan owns its device memory and
+ *     one HIP stream.  One host thread may use a plan at a time.
+ *   - State layout on the host: heat and KS take nx REALs; Euler takes nx
+ *     cells of three REALs (rho, rho*u, E) interleaved, as the reference's
+ *     REALthree arrays (Euler1D_SweptShared.cu:1175-1179).
+ *   - REAL is double for SW1D_F64 plans and float for SW1D_F32 plans.
+ */
+#ifndef SWEPT1D_H
+#define SWEPT1D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SW1D_ABI_VERSION 1
+
+enum {
+    SW1D_OK = 0,
+    SW1D_EINVAL = -1,     /* bad argument (validation messages of main(), Heat:700-740) */
+    SW1D_EDEVICE = -2,    /* HIP runtime error (cudaGetLastError path, Heat:796-802) */
+    SW1D_ENOMEM = -3,     /* device or pinned allocation failed */
+    SW1D_EUNSTABLE = -4,  /* Euler dt/dx > 0.21 (Euler:1168-1172) */
+    SW1D_ENOTSUP = -5     /* combination the build does not provide */
+};
+
+/* Equations: one reference program each. */
+enum { SW1D_HEAT = 0, SW1D_KS = 1, SW1D_EULER = 2 };
+/* Precision: the -DREAL=double build (Makefile:4) or the float default. */
+enum { SW1D_F64 = 0, SW1D_F32 = 1 };
+/* Algorithm = argv[6] of the reference: 0 classic, 1 shared swept,
+ * 2 hybrid CPU+GPU swept (heat, Euler) / register swept (KS). */
+enum { SW1D_CLASSIC = 0, SW1D_SWEPT = 1, SW1D_ALT = 2 };
+/* Boundary conditions.  DEFAULT picks the reference's: heat insulated
+ * (mirror, Heat:140-142), KS periodic (KS:199), Euler Dirichlet ghost cells
+ * (Euler:405-416).  PERIODIC heat is the synthetic periodic-heat workload. */
+enum { SW1D_BC_DEFAULT = 0, SW1D_BC_MIRROR = 1, SW1D_BC_PERIODIC = 2, SW1D_BC_DIRICHLET = 3 };
+
+typedef struct sw1d_problem {
+    int32_t equation;    /* SW1D_HEAT / SW1D_KS / SW1D_EULER */
+    int32_t precision;   /* SW1D_F64 / SW1D_F32 */
+    int32_t algorithm;   /* SW1D_CLASSIC / SW1D_SWEPT / SW1D_ALT */
+    int32_t bc;          /* SW1D_BC_* */
+    int64_t nx;          /* spatial points, argv[1] */
+    int32_t tpb;         /* threads per block, argv[2]: the swept node width */
+    int32_t node_width;  /* 0: derive from tpb; else points per swept node (64*2^k) */
+    double dt;           /* time step, argv[3] */
+    double dx;           /* 0: reference value (heat ds 0.001, KS 0.5, Euler 1/(nx-2)) */
+    double alpha;        /* heat diffusivity; 0: reference 8.418e-5 (Heat:56) */
+} sw1d_problem;
+
+typedef struct sw1d_plan sw1d_plan;
+
+/* Library / device ------------------------------------------------------- */
+int sw1d_abi_version(void);
+const char* sw1d_last_error(void);
+/* Runs the lane-shift self check the swept kernels depend on.  1 = pass. */
+int sw1d_device_check(void);
+
+/* Plan lifecycle ---------------------------------------------------------- */
+/* Replaces the setup of main() + the cudaMalloc block of classicWrapper /
+ * sweptWrapper (Heat:471-481, 523-538; KS:596-605, 660-672; Euler:839-849,
+ * 901-921): validates the problem, computes the constants exactly as the
+ * reference main() does (Heat:719-728, KS:917-926, Euler:1127-1154) and
+ * allocates the device state. */
+int sw1d_plan_create(sw1d_plan** plan, const sw1d_problem* problem);
+int sw1d_plan_destroy(sw1d_plan* plan);
+
+/* H2D of the initial condition (Heat:481/538, KS:605/672, Euler:849/921). */
+int sw1d_set_state(sw1d_plan* plan, const void* host);
+/* Synchronous D2H of the current state (Heat:499/514/623/686 …). */
+int sw1d_get_state(sw1d_plan* plan, void* host);
+/* Device-to-device forms for callers that keep the state in HBM
+ * (same layout as the host forms). */
+int sw1d_set_state_device(sw1d_plan* plan, const void* device_ptr);
+int sw1d_get_state_device(sw1d_plan* plan, void* device_ptr);
+
+/* Advance the state by nsteps time steps with the plan's algorithm.  Replaces
+ * the kernel loops of classicWrapper (Heat:491-510, KS:612-631,
+ * Euler:859-888) and sweptWrapper (Heat:543-682, KS:753-851,
+ * Euler:927-1100) between two outputs.  Asynchronous on the plan's stream
+ * for the GPU-only algorithms; the hybrid algorithm synchronises once per
+ * CPU node.  Results are bitwise the same for every algorithm. */
+int sw1d_advance(sw1d_plan* plan, int64_t nsteps);
+int sw1d_synchronize(sw1d_plan* plan);
+/* Advance and return the elapsed device time in milliseconds (hipEvents on
+ * the plan's stream; replaces the cudaEvent pair of Heat:774-794). */
+int sw1d_advance_timed(sw1d_plan* plan, int64_t nsteps, double* elapsed_ms);
+
+/* Plan facts ----------------------------------------------------------------
+ * info[0] node width (points per swept node), info[1] steps advanced by one
+ * swept tile launch, info[2] values stored per point (1 heat, 2 KS, 6 Euler),
+ * info[3] device bytes held by the plan, info[4] steps per reference cycle
+ * (the t_fullstep of sweptWrapper / classicWrapper divided by dt). */
+int sw1d_plan_info(const sw1d_plan* plan, int64_t info[5]);
+/* The REAL constants the kernels use, widened to double:
+ * heat {Fo, 1-2Fo}; KS {1/(4dx), 1/dx^2, 1/dx^4, dt, dt/2};
+ * Euler {gam, mgam, dt/dx, rhoL, mL, EL, rhoR, mR, ER}. */
+int sw1d_plan_constants(const sw1d_plan* plan, double* out, int32_t capacity);
+/* hipStream_t of the plan, as void*. */
+int sw1d_plan_stream(const sw1d_plan* plan, void** stream);
+
+/* Per-kernel timing: when enabled, every launch of the plan is bracketed by
+ * hipEvents on the plan's stream.  kind: 0 classic stage, 1 up-triangle,
+ * 2 down-triangle, 3 diamond, 4 split (boundary-straddling) diamond. */
+int sw1d_set_profiling(sw1d_plan* plan, int32_t enabled);
+int sw1d_kernel_stats(sw1d_plan* plan, int32_t kind, int64_t* launches, double* total_ms);
+
+/* Schedule ------------------------------------------------------------------
+ * The reference's time bookkeeping (Heat:483-510, 541-682; KS:609-631,
+ * 676-851; Euler:855-888, 924-1098): given the argv values it yields the
+ * step counts at which the program writes a .dat row and the time it prints
+ * there.  out_steps/out_times receive up to capacity rows (the last row is the
+ * final state, row count returned in *count).  Returns SW1D_EINVAL if
+ * capacity is too small (count then holds the needed size).
+ * kind: 0 heat classic, 1 heat swept/hybrid, 2 KS classic, 3 KS swept,
+ * 4 KS register, 5 Euler classic, 6 Euler swept/hybrid. */
+int sw1d_schedule(int32_t kind, double dt, double tf_arg, double freq, int32_t tpb,
+                  double dt_sched, int64_t* out_steps, double* out_times,
+                  int64_t capacity, int64_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SWEPT1D_H */

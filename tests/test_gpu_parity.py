@@ -1,0 +1,103 @@
+"""GPU parity: every algorithm of libswept1d against the CPU oracle, bitwise.
+
+The reference's invariant is swept == classic (ResultPlots/ExactTesting/
+consistency.out); the build holds every path — classic, swept, hybrid /
+register — to the same bits as the serial oracle, which at the chaotic KS
+horizon is the only way to stay within 1e-12 relative.  Calls go through the
+C ABI (ctypes on lib/libswept1d.so)."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _ks_u0(orc, n, dtype):
+    return orc.ic_ks_noisy(n, 11).astype(dtype)
+
+
+def _oracle(orc, eq, u0, nsteps, dt, dtype, bc, n):
+    if eq == 0:
+        fo, fo2 = orc.heat_consts(dt, dtype)
+        return orc.heat_run(u0, nsteps, fo, fo2, 0 if bc != 2 else 1)
+    if eq == 1:
+        return orc.ks_run(u0, nsteps, orc.ks_consts(dt, dtype))
+    k, bd = orc.euler_consts(dt, n, dtype)
+    return orc.euler_run(u0, nsteps, k, bd)
+
+
+def _ic(orc, eq, n, dtype, bc):
+    if eq == 0:
+        return orc.heat_ic_ref(n, dtype) if bc != 2 else orc.ic_heat_fourier(n, 5)[0].astype(dtype)
+    if eq == 1:
+        return _ks_u0(orc, n, dtype)
+    return orc.euler_ic_ref(n, dtype)
+
+
+def _dt(eq, n, bc):
+    if eq == 0:
+        return 0.001 if bc != 2 else 0.0025
+    if eq == 1:
+        return 0.005
+    return 0.1 / (n - 2)
+
+
+def test_device_lane_shift_check(sw):
+    _say("device_check")
+    assert sw.device_check()
+
+
+CASES = []
+for eq, bc in ((0, 0), (0, 2), (1, 0), (2, 0)):
+    for prec in (0, 1):
+        for alg in (0, 1, 2):
+            CASES.append((eq, bc, prec, alg))
+
+
+@pytest.mark.parametrize("eq,bc,prec,alg", CASES)
+def test_parity_vs_oracle(sw, orc, eq, bc, prec, alg):
+    n = 4096
+    dtype = np.float64 if prec == 0 else np.float32
+    dt = _dt(eq, n, bc)
+    tpb = 256
+    nsteps = 300 if eq != 2 else 200  # not a multiple of the tile height: remainder path too
+    _say(f"parity eq={eq} bc={bc} prec={prec} alg={alg} n={n} tpb={tpb} steps={nsteps}")
+    u0 = _ic(orc, eq, n, dtype, bc)
+    with sw.Plan(eq, n, dt, precision=prec, algorithm=alg, tpb=tpb, bc=bc) as p:
+        p.set_state(u0)
+        p.advance(nsteps // 3)
+        p.advance(nsteps - nsteps // 3)
+        got = p.get_state()
+    want = _oracle(orc, eq, u0, nsteps, dt, dtype, bc, n)
+    assert got.dtype == want.dtype
+    bad = np.flatnonzero(got != want)
+    assert bad.size == 0, f"{bad.size} mismatches, first at {bad[:8]}: {got[bad[:4]]} vs {want[bad[:4]]}"
+
+
+@pytest.mark.parametrize("eq", [0, 1, 2])
+@pytest.mark.parametrize("W", [64, 128, 256, 512, 1024])
+def test_node_widths(sw, orc, eq, W):
+    if eq != 0 and W < 128:
+        pytest.skip("radius-2 stages need >= 2 points per lane")
+    if eq == 2 and W > 512:
+        pytest.skip("Euler nodes stop at 512 points")
+    n = 8 * W
+    dtype = np.float64
+    dt = _dt(eq, n, 0)
+    H = W // 2 if eq == 0 else W // 8
+    nsteps = 5 * H + 3
+    _say(f"node width eq={eq} W={W} n={n} steps={nsteps}")
+    u0 = _ic(orc, eq, n, dtype, 0)
+    with sw.Plan(eq, n, dt, algorithm=1, node_width=W) as p:
+        assert p.info()["node_width"] == W
+        p.set_state(u0)
+        p.advance(nsteps)
+        got = p.get_state()
+    want = _oracle(orc, eq, u0, nsteps, dt, dtype, 0, n)
+    assert np.array_equal(got, want)
