@@ -1,0 +1,167 @@
+"""Benchmark: point-updates/s of the 1D swept-rule solver on one MI355X.
+
+Workload (default): Kuramoto-Sivashinsky, FP64, nX = 2^20, dx = 0.5,
+dt = 0.005, IC 2cos(19 pi x/128) + N(0, 1e-3) noise (seed 1), shared swept
+algorithm.  A "step" is one time step of the whole domain (nX point-updates);
+the timed region is exactly --steps steps with the state already in HBM,
+bracketed by device synchronisation.  The classic algorithm (one launch per RK
+stage) is timed on the same input for the swept/classic ratio.
+
+The roofline entry is for the dominant kernel (the swept diamond): its
+average duration comes from hipEvents recorded around every launch on the
+plan's stream during a second pass over the same steps; achieved = algorithmic
+FP64 flops per launch (40 flop per KS point-update x nX x W/8 steps) / that
+duration.
+
+The CPU baseline is the serial CPU oracle (one core) on a bounded sample of
+the same workload (same nX, fewer steps), timed on this host.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+KS_FLOP_PER_PU = 40          # stutterStep 20 + finalStep 20 (KS1D_SweptShared.cu:159-171)
+KS_SWEPT_BYTES_PER_PU = 256  # per tile launch: (u, u_half) read + written, 32 B / (W/8 steps) * W
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+
+def _timed(plan, steps):
+    plan.synchronize()
+    t0 = time.perf_counter()
+    plan.advance(steps)
+    plan.synchronize()
+    return time.perf_counter() - t0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--warmup", type=int, default=2000)
+    ap.add_argument("--nx", type=int, default=1 << 20)
+    ap.add_argument("--node-width", type=int, default=512)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(the solver runs on a single device, no collectives)", file=sys.stderr)
+        sys.exit(2)
+
+    sw = importlib.import_module("1dsweptcuda_amd")
+    n, dt = args.nx, 0.005
+    rng_ic = _ks_ic(n, args.seed)
+
+    # ---------------------------------------------------------- swept (alg 1)
+    plan = sw.Plan(sw.KS, n, dt, algorithm=sw.SWEPT, tpb=args.node_width, node_width=args.node_width)
+    info = plan.info()
+    plan.set_state(rng_ic)
+    plan.advance(args.warmup)
+    secs = _timed(plan, args.steps)
+    value = n * args.steps / secs
+
+    # per-kernel durations over the same number of steps
+    plan.set_profiling(True)
+    plan.advance(args.steps)
+    plan.synchronize()
+    dn, dms = plan.kernel_stats(sw.K_DIAMOND)
+    sn, sms = plan.kernel_stats(sw.K_SPLIT)
+    plan.set_profiling(False)
+    launches = dn + sn
+    avg_ms = (dms + sms) / max(launches, 1)
+    W = info["node_width"]
+    pu_per_launch = n * (W // 8)
+    flops = KS_FLOP_PER_PU * pu_per_launch
+    achieved_tf = flops / (avg_ms * 1e-3) / 1e12
+    plan.close()
+
+    # ------------------------------------------------------- classic (alg 0)
+    cplan = sw.Plan(sw.KS, n, dt, algorithm=sw.CLASSIC, tpb=256)
+    cplan.set_state(rng_ic)
+    cplan.advance(min(args.warmup, 500))
+    csteps = max(args.steps // 4, 1)
+    csecs = _timed(cplan, csteps)
+    classic_value = n * csteps / csecs
+    cplan.set_profiling(True)
+    cplan.advance(200)
+    cplan.synchronize()
+    cn, cms = cplan.kernel_stats(sw.K_CLASSIC)
+    cplan.close()
+    classic_stage_us = cms / max(cn, 1) * 1e3
+    # stage bytes: predictor reads u writes u_half (16 B), corrector reads
+    # u_half and u, writes u (24 B): 40 B per point-update over two launches
+    classic_gbs = 40 * n / (2 * classic_stage_us * 1e-6) / 1e9
+
+    out = {
+        "metric": "point-updates/s (KS FP64 swept, nX=2^20, bitwise equal to the FP64 CPU oracle)",
+        "value": value,
+        "unit": "point-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": secs / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded IC 2cos(19 pi x/128) + N(0,1e-3), seed %d)" % args.seed,
+        "config": {"workload": "KS 1D, RK2 + central differences, FP64, periodic",
+                   "nx": n, "dx": 0.5, "dt": dt, "algorithm": "shared swept (alg 1)",
+                   "node_width": W, "steps_per_tile_launch": W // 8},
+        "classic": {"value": classic_value, "ms_per_step": csecs / csteps * 1e3,
+                    "stage_kernel_us": classic_stage_us, "stage_hbm_gbs": classic_gbs},
+        "swept_over_classic": value / classic_value,
+        "roofline": {"bound": "mfma", "compute": "fp64 vector ALU (no MFMA); peak = MI355X FP64",
+                     "kernel": "k_ks_swept<double,%d,DIAMOND>" % (W // 64),
+                     "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved_tf / FP64_PEAK_TFLOPS, "traffic": None,
+                     "avg_launch_us": avg_ms * 1e3, "launches": launches,
+                     "flop_per_launch": flops,
+                     "algorithmic_bytes_per_launch": 32 * n},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = _cpu_baseline(n, dt, rng_ic, args.cpu_seconds)
+    print(json.dumps(out))
+
+
+def _ks_ic(n, seed):
+    # same formula as the oracle's synthetic IC; bench needs no oracle for it
+    x = np.arange(n) * 0.5
+    base = 2.0 * np.cos(19.0 * x * np.pi / 128.0)
+    rng = np.random.default_rng(seed)
+    return base + 1e-3 * rng.standard_normal(n)
+
+
+def _cpu_baseline(n, dt, u0, budget_s):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+
+    c = pyoracle.ks_consts(dt)
+    t0 = time.perf_counter()
+    pyoracle.ks_run(u0, 2, c)
+    per_step = (time.perf_counter() - t0) / 2
+    steps = max(2, int(budget_s / max(per_step, 1e-9)))
+    t0 = time.perf_counter()
+    pyoracle.ks_run(u0, steps, c)
+    secs = time.perf_counter() - t0
+    return {"value": n * steps / secs, "unit": "point-updates/s", "cores": 1, "kind": "port",
+            "sample": f"KS FP64 nX={n}, {steps} steps, serial C oracle (oracle/swept1d_oracle.c)"}
+
+
+if __name__ == "__main__":
+    main()

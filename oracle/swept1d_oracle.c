@@ -34,15 +34,23 @@
 
 #define REAL double
 #define SFX _d
+#define RSQRT sqrt
+#define RFABS fabs
 #include "oracle_body.inc"
 #undef REAL
 #undef SFX
+#undef RSQRT
+#undef RFABS
 
 #define REAL float
 #define SFX _f
+#define RSQRT sqrtf
+#define RFABS fabsf
 #include "oracle_body.inc"
 #undef REAL
 #undef SFX
+#undef RSQRT
+#undef RFABS
 
 /* ------------------------------------------------ seeded synthetic ICs --- */
 /* splitmix64 (Steele, Lea, Flood 2014), a portable generator so that the same
