@@ -24,7 +24,7 @@ HEADER = os.path.join(os.path.dirname(_HERE), "include", "swept1d.h")
 
 HEAT, KS, EULER = 0, 1, 2
 F64, F32 = 0, 1
-CLASSIC, SWEPT, ALT = 0, 1, 2
+CLASSIC, SWEPT, ALT, TRAPEZOID = 0, 1, 2, 3
 BC_DEFAULT, BC_MIRROR, BC_PERIODIC, BC_DIRICHLET = 0, 1, 2, 3
 
 # sw1d_schedule kinds (Heat / KS / Euler classic and swept bookkeeping).
@@ -56,6 +56,8 @@ class Problem(C.Structure):
         ("dt", C.c_double),
         ("dx", C.c_double),
         ("alpha", C.c_double),
+        ("tile_depth", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 
@@ -137,8 +139,9 @@ class Plan:
 
     def __init__(self, equation: int, nx: int, dt: float, *, precision: int = F64,
                  algorithm: int = SWEPT, tpb: int = 64, node_width: int = 0,
-                 bc: int = BC_DEFAULT, dx: float = 0.0, alpha: float = 0.0):
-        self.prob = Problem(equation, precision, algorithm, bc, nx, tpb, node_width, dt, dx, alpha)
+                 bc: int = BC_DEFAULT, dx: float = 0.0, alpha: float = 0.0, tile_depth: int = 0):
+        self.prob = Problem(equation, precision, algorithm, bc, nx, tpb, node_width, dt, dx, alpha,
+                            tile_depth, 0)
         self.dtype = _dtype(precision)
         self.equation = equation
         self.nx = nx
@@ -224,3 +227,75 @@ def program(equation: int, precision: int) -> str:
     """Path of the drop-in executable, e.g. bin/KSDoubleOut."""
     name = ["Heat", "KS", "Euler"][equation] + ("Double" if precision == F64 else "Single") + "Out"
     return os.path.join(BIN_DIR, name)
+
+
+class Batch:
+    """Many independent periodic heat / KS problems advanced by one persistent
+    launch (sw1d_batch_*).  State is the concatenation of the fields."""
+
+    def __init__(self, equations, nx, dt, dx, *, precision: int = F64, tile_points=None):
+        L = lib()
+        P = C.c_void_p
+        L.sw1d_batch_create.argtypes = [C.POINTER(P), C.c_int32, C.c_int64, P, P, P, P, P]
+        L.sw1d_batch_destroy.argtypes = [P]
+        L.sw1d_batch_set_state.argtypes = [P, P]
+        L.sw1d_batch_get_state.argtypes = [P, P]
+        L.sw1d_batch_advance.argtypes = [P, C.c_int64]
+        L.sw1d_batch_advance_timed.argtypes = [P, C.c_int64, C.POINTER(C.c_double)]
+        L.sw1d_batch_info.argtypes = [P, C.POINTER(C.c_int64)]
+        self.eq = np.ascontiguousarray(equations, np.int32)
+        self.nx = np.ascontiguousarray(nx, np.int64)
+        self.dt = np.ascontiguousarray(dt, np.float64)
+        self.dx = np.ascontiguousarray(dx, np.float64)
+        self.tp = None if tile_points is None else np.ascontiguousarray(tile_points, np.int32)
+        self.dtype = _dtype(precision)
+        self.offsets = np.concatenate([[0], np.cumsum(self.nx)])
+        self._h = C.c_void_p()
+        ptr = lambda a: a.ctypes.data_as(P) if a is not None else None
+        _check(L.sw1d_batch_create(C.byref(self._h), precision, len(self.nx), ptr(self.eq),
+                                   ptr(self.nx), ptr(self.dt), ptr(self.dx), ptr(self.tp)),
+               "sw1d_batch_create")
+
+    def close(self):
+        if self._h:
+            lib().sw1d_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, packed) -> None:
+        a = np.ascontiguousarray(packed, dtype=self.dtype)
+        if a.size != self.offsets[-1]:
+            raise ValueError("packed state has the wrong length")
+        _check(lib().sw1d_batch_set_state(self._h, a.ctypes.data_as(C.c_void_p)), "sw1d_batch_set_state")
+
+    def get_state(self) -> np.ndarray:
+        out = np.empty(int(self.offsets[-1]), self.dtype)
+        _check(lib().sw1d_batch_get_state(self._h, out.ctypes.data_as(C.c_void_p)), "sw1d_batch_get_state")
+        return out
+
+    def advance(self, nsteps: int) -> None:
+        _check(lib().sw1d_batch_advance(self._h, nsteps), "sw1d_batch_advance")
+
+    def advance_timed(self, nsteps: int) -> float:
+        ms = C.c_double(0)
+        _check(lib().sw1d_batch_advance_timed(self._h, nsteps, C.byref(ms)), "sw1d_batch_advance_timed")
+        return ms.value
+
+    def info(self) -> dict:
+        v = (C.c_int64 * 4)()
+        _check(lib().sw1d_batch_info(self._h, v), "sw1d_batch_info")
+        return {"total_points": v[0], "workgroups": v[1], "problems": v[2], "device_bytes": v[3]}
+
+    def split(self, packed):
+        return [packed[self.offsets[i]:self.offsets[i + 1]] for i in range(len(self.nx))]

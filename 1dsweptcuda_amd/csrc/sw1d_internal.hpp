@@ -43,7 +43,44 @@ struct Coefs {
     EulerCoef<double> eu_d; EulerCoef<float> eu_f;
 };
 
-// Launchers (sw1d_kernels.hip).  All return a hipError_t.
+// One trapezoid launch: tile t stores the points [t*W_out, t*W_out + W_out)
+// (the last tile clipped at nX) after S stages, W_out = 64*PO, from a window
+// of 64*(PO+SB) points; S = 32*SB stages for heat and 16*SB for KS / Euler.
+struct TrapLaunch {
+    int eq, bc, PO, SB;
+    int64_t n, ntiles, tile_first;
+    const void* in;
+    void* out;
+};
+
+// One problem of a batch.  Periodic heat (c = {Fo, 1-2Fo}) or KS
+// (c = {1/(4dx), 1/dx^2, 1/dx^4, dt, dt/2}), stored at `offset` in the packed
+// state arrays, advanced with trapezoid tiles of 64*po output points.
+template <typename T> struct BatchJob {
+    int64_t offset;
+    int32_t n, eq, po, pad;
+    T c[5];
+};
+
+struct BatchLaunch {
+    const void* jobs;  // device BatchJob<T>[njobs], in queue order
+    int32_t njobs;
+    int* next;         // device work-queue counter (set to 0 by the launch)
+    void* a;           // packed state; receives the result
+    void* b;           // packed scratch of the same size
+    int64_t nsteps;
+    int32_t grid;      // persistent workgroups
+};
+
+// Launchers (sw1d_kernels.hip, sw1d_trapezoid.hip, sw1d_batch.hip).  All
+// return a hipError_t.
+hipError_t launch_batch(const BatchLaunch& L, bool f64, hipStream_t s);
+int batch_blocks_per_cu(bool f64);
+constexpr int kBatchThreads = 256;
+inline bool batch_tile_supported(int po) { return po == 2 || po == 4 || po == 8; }
+hipError_t launch_trapezoid(const TrapLaunch& L, bool f64, const Coefs& c, hipStream_t s);
+bool trapezoid_supported(int eq, int PO, int SB);
+inline int64_t trapezoid_steps(int eq, int SB) { return eq == EQ_HEAT ? 32 * SB : 8 * SB; }
 hipError_t launch_swept(const SweptLaunch& L, bool f64, const Coefs& c, hipStream_t s);
 // One classic stage: heat (stage ignored), KS / Euler stage 0 = predictor,
 // 1 = corrector.  State layout as above; heat ping-pongs between in and out,

@@ -33,62 +33,15 @@
 //   (classicHeat Heat:132-145, classicKS KS:189-207, classicEuler Euler:400-426).
 
 #include "sw1d_internal.hpp"
+#include "sw1d_lanes.hpp"
 
 namespace sw1d {
 namespace {
 
-constexpr int kWave = 64;
 constexpr int kWavesPerBlock = 4;
 constexpr int kBlock = kWave * kWavesPerBlock;
 constexpr int kClassicBlock = 256;
 
-// ---------------------------------------------------------- lane shifts ---
-// DPP wave_shr:1 (0x138): lane l receives lane l-1; lane 0 keeps `old`.
-// DPP wave_shl:1 (0x130): lane l receives lane l+1; lane 63 keeps `old`.
-__device__ __forceinline__ int shr1_i(int v, int old)
-{
-    return __builtin_amdgcn_update_dpp(old, v, 0x138, 0xf, 0xf, false);
-}
-__device__ __forceinline__ int shl1_i(int v, int old)
-{
-    return __builtin_amdgcn_update_dpp(old, v, 0x130, 0xf, 0xf, false);
-}
-
-__device__ __forceinline__ double from_left(double v, double old)
-{
-    const int lo = shr1_i(__double2loint(v), __double2loint(old));
-    const int hi = shr1_i(__double2hiint(v), __double2hiint(old));
-    return __hiloint2double(hi, lo);
-}
-__device__ __forceinline__ double from_right(double v, double old)
-{
-    const int lo = shl1_i(__double2loint(v), __double2loint(old));
-    const int hi = shl1_i(__double2hiint(v), __double2hiint(old));
-    return __hiloint2double(hi, lo);
-}
-__device__ __forceinline__ float from_left(float v, float old)
-{
-    return __int_as_float(shr1_i(__float_as_int(v), __float_as_int(old)));
-}
-__device__ __forceinline__ float from_right(float v, float old)
-{
-    return __int_as_float(shl1_i(__float_as_int(v), __float_as_int(old)));
-}
-
-// e[0 .. P+2R-1] = values at node-local x = l*P - R .. l*P + P + R - 1.
-template <typename T, int P, int R>
-__device__ __forceinline__ void gather(const T (&v)[P], const T (&hl)[R], const T (&hr)[R],
-                                       T (&e)[P + 2 * R])
-{
-    static_assert(P >= R, "a lane must hold at least one stencil radius");
-#pragma unroll
-    for (int p = 0; p < P; ++p) e[R + p] = v[p];
-#pragma unroll
-    for (int i = 0; i < R; ++i) {
-        e[R - 1 - i] = from_left(v[P - 1 - i], hl[i]);
-        e[R + P + i] = from_right(v[i], hr[i]);
-    }
-}
 
 template <typename T, int P>
 __device__ __forceinline__ void load_pts(const T* __restrict__ a, int64_t i0, T (&v)[P])
@@ -278,6 +231,8 @@ __global__ __launch_bounds__(kBlock) void k_euler_swept(const T* __restrict__ in
 
     // src = 0: predictor from the conserved values (Euler:307-350);
     // src = 3: corrector from the predictor values (Euler:353-389, :420).
+    // Each interface flux is evaluated once and used by the two cells beside
+    // it (P+1 interfaces per lane instead of 2P).
     auto stage = [&](int thr, auto src_tag) {
         constexpr int src = decltype(src_tag)::value;
         constexpr int dst = 3 - src;
@@ -285,26 +240,37 @@ __global__ __launch_bounds__(kBlock) void k_euler_swept(const T* __restrict__ in
         gather<T, P, 2>(q[src + 0], hl[src + 0], hr[src + 0], ex);
         gather<T, P, 2>(q[src + 1], hl[src + 1], hr[src + 1], ey);
         gather<T, P, 2>(q[src + 2], hl[src + 2], hr[src + 2], ez);
+        auto cell = [&](int j) { return Q3<T>{ex[j], ey[j], ez[j]}; };
+        // pressure differences between ext cells j and j+1
+        T pd[P + 3];
+#pragma unroll
+        for (int j = 0; j < P + 3; ++j) pd[j] = eu_pdiff(cell(j), cell(j + 1));
+        // interface j lies between ext cells j+1 and j+2 (cells lb+j-1, lb+j);
+        // the interface left of global cell 1 drops pdiff(-1,0), the one right
+        // of cell n-2 drops pdiff(n-1,n) (the flags of Euler:405, :424).
+        auto iface = [&](int j, Q3<T>& F, Q3<T>& D) {
+            const T q0 = (j == 1 && g0) ? T(0) : pd[j];
+            const T q2 = (j == P - 1 && gN) ? T(0) : pd[j + 2];
+            eu_iface(cell(j), cell(j + 1), cell(j + 2), cell(j + 3), q0, pd[j + 1], q2, k, F, D);
+        };
+        const T c = src == 0 ? T(0.25) * k.dtdx : T(0.5) * k.dtdx;
+        Q3<T> FL, DL;
+        iface(0, FL, DL);
 #pragma unroll
         for (int p = 0; p < P; ++p) {
-            const Q3<T> s0{ex[p], ey[p], ez[p]}, s1{ex[p + 1], ey[p + 1], ez[p + 1]};
-            const Q3<T> s2{ex[p + 2], ey[p + 2], ez[p + 2]}, s3{ex[p + 3], ey[p + 3], ez[p + 3]};
-            const Q3<T> s4{ex[p + 4], ey[p + 4], ez[p + 4]};
-            const bool fl = (p == 1) ? g0 : false;
-            const bool fr = (p == P - 2) ? gN : false;
-            Q3<T> nv;
-            if (src == 0) {
-                nv = eu_predict(s0, s1, s2, s3, s4, fl, fr, k);
-            } else {
-                const Q3<T> uc{q[0][p], q[1][p], q[2][p]};
-                nv = eu_correct(uc, s0, s1, s2, s3, s4, fl, fr, k);
-            }
+            Q3<T> FR, DR;
+            iface(p + 1, FR, DR);
+            const Q3<T> f = eu_balance(FL, DL, FR, DR);
+            const Q3<T> base = src == 0 ? cell(p + 2) : Q3<T>{q[0][p], q[1][p], q[2][p]};
+            Q3<T> nv = eu_apply(base, f, c);
             if (p == 0 && g0) nv = k.bl;
             if (p == P - 1 && gN) nv = k.br;
             const bool act = d[p] <= thr;
             q[dst + 0][p] = act ? nv.x : q[dst + 0][p];
             q[dst + 1][p] = act ? nv.y : q[dst + 1][p];
             q[dst + 2][p] = act ? nv.z : q[dst + 2][p];
+            FL = FR;
+            DL = DR;
         }
     };
     using A = std::integral_constant<int, 0>;

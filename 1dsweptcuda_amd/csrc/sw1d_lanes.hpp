@@ -1,0 +1,59 @@
+// Wave64 lane-shift primitives shared by the register-resident tile kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace sw1d {
+
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------- lane shifts ---
+// DPP wave_shr:1 (0x138): lane l receives lane l-1; lane 0 keeps `old`.
+// DPP wave_shl:1 (0x130): lane l receives lane l+1; lane 63 keeps `old`.
+__device__ __forceinline__ int shr1_i(int v, int old)
+{
+    return __builtin_amdgcn_update_dpp(old, v, 0x138, 0xf, 0xf, false);
+}
+__device__ __forceinline__ int shl1_i(int v, int old)
+{
+    return __builtin_amdgcn_update_dpp(old, v, 0x130, 0xf, 0xf, false);
+}
+
+__device__ __forceinline__ double from_left(double v, double old)
+{
+    const int lo = shr1_i(__double2loint(v), __double2loint(old));
+    const int hi = shr1_i(__double2hiint(v), __double2hiint(old));
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double from_right(double v, double old)
+{
+    const int lo = shl1_i(__double2loint(v), __double2loint(old));
+    const int hi = shl1_i(__double2hiint(v), __double2hiint(old));
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float from_left(float v, float old)
+{
+    return __int_as_float(shr1_i(__float_as_int(v), __float_as_int(old)));
+}
+__device__ __forceinline__ float from_right(float v, float old)
+{
+    return __int_as_float(shl1_i(__float_as_int(v), __float_as_int(old)));
+}
+
+// e[0 .. P+2R-1] = values at node-local x = l*P - R .. l*P + P + R - 1.
+template <typename T, int P, int R>
+__device__ __forceinline__ void gather(const T (&v)[P], const T (&hl)[R], const T (&hr)[R],
+                                       T (&e)[P + 2 * R])
+{
+    static_assert(P >= R, "a lane must hold at least one stencil radius");
+#pragma unroll
+    for (int p = 0; p < P; ++p) e[R + p] = v[p];
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        e[R - 1 - i] = from_left(v[P - 1 - i], hl[i]);
+        e[R + P + i] = from_right(v[i], hr[i]);
+    }
+}
+
+
+}  // namespace sw1d

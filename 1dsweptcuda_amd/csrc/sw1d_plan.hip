@@ -51,6 +51,7 @@ struct sw1d_plan {
     size_t S = 8;      // sizeof(REAL)
     int nc = 1;        // values per point
     int W = 0, P = 0;  // swept node width, points per lane (0: no swept path)
+    int SB = 0;        // trapezoid: extra points per lane (stages per launch / (32/R))
     int R = 1;         // stage radius
     int64_t tile_steps = 0;  // steps advanced by one swept tile launch
     int64_t cycle_steps = 0; // reference t_fullstep / dt
@@ -420,6 +421,34 @@ int swept_advance(sw1d_plan* p, int64_t nsteps)
     return SW1D_OK;
 }
 
+int trapezoid_advance(sw1d_plan* p, int64_t nsteps)
+{
+    const int64_t H = p->tile_steps;
+    const int64_t launches = nsteps / H;
+    TrapLaunch L;
+    L.eq = p->eq;
+    L.bc = p->bc;
+    L.PO = p->P;
+    L.SB = p->SB;
+    L.n = p->n;
+    L.ntiles = (p->n + p->W - 1) / p->W;
+    L.tile_first = 0;
+    for (int64_t i = 0; i < launches; ++i) {
+        L.in = p->buf[p->cur];
+        L.out = p->buf[p->cur ^ 1];
+        {
+            ProfScope ps(p, KK_DIAMOND);
+            HIP_OR_FAIL(launch_trapezoid(L, p->f64, p->coef, p->stream));
+        }
+        p->cur ^= 1;
+    }
+    for (int64_t s = launches * H; s < nsteps; ++s) {
+        int rc = classic_step(p);
+        if (rc) return rc;
+    }
+    return SW1D_OK;
+}
+
 void release(sw1d_plan* p)
 {
     if (!p) return;
@@ -466,7 +495,7 @@ int sw1d_plan_create(sw1d_plan** out, const sw1d_problem* q)
         return fail(SW1D_EINVAL, "unknown equation %d", q->equation);
     if (q->precision != SW1D_F64 && q->precision != SW1D_F32)
         return fail(SW1D_EINVAL, "unknown precision %d", q->precision);
-    if (q->algorithm < SW1D_CLASSIC || q->algorithm > SW1D_ALT)
+    if (q->algorithm < SW1D_CLASSIC || q->algorithm > SW1D_TRAPEZOID)
         return fail(SW1D_EINVAL, "unknown algorithm %d", q->algorithm);
     if (q->nx < 8) return fail(SW1D_EINVAL, "nx must be >= 8 (got %lld)", (long long)q->nx);
     if (!(q->dt > 0) || !std::isfinite(q->dt)) return fail(SW1D_EINVAL, "dt must be positive");
@@ -501,11 +530,35 @@ int sw1d_plan_create(sw1d_plan** out, const sw1d_problem* q)
     // Reference cycle: t_fullstep / dt (Heat:483/541, KS:616/676/755, Euler:863/924).
     const int tpb = q->tpb > 0 ? q->tpb : 64;
     if (p->alg == SW1D_CLASSIC) p->cycle_steps = p->eq == EQ_HEAT ? 2 : 1;
+    else if (p->alg == SW1D_TRAPEZOID) p->cycle_steps = 0;
     else if (p->eq == EQ_HEAT) p->cycle_steps = tpb;
     else if (p->eq == EQ_KS && p->alg == SW1D_ALT) p->cycle_steps = 8;
     else p->cycle_steps = tpb / 4;
 
-    if (p->alg != SW1D_CLASSIC) {
+    if (p->alg == SW1D_TRAPEZOID) {
+        const int SB = q->tile_depth > 0 ? q->tile_depth : 1;
+        int W = q->node_width > 0 ? q->node_width
+              : (p->eq == EQ_HEAT ? 1024 : (p->eq == EQ_KS ? 512 : 256));
+        if (!is_pow2(W) || W < 128 || !trapezoid_supported(p->eq, W / 64, SB)) {
+            delete p;
+            return fail(SW1D_ENOTSUP, "no trapezoid tile of %d points and depth %d for this equation",
+                        W, SB);
+        }
+        if (p->n < 2 * (int64_t)W) {
+            delete p;
+            return fail(SW1D_ENOTSUP, "nx=%lld is below two trapezoid tiles of %d points",
+                        (long long)p->n, W);
+        }
+        p->W = W;
+        p->P = W / 64;
+        p->SB = SB;
+        p->tile_steps = trapezoid_steps(p->eq, SB);
+        if (g_lane_check == 0) sw1d_device_check();
+        if (g_lane_check != 1) {
+            delete p;
+            return fail(SW1D_EDEVICE, "DPP lane-shift self check failed on this device");
+        }
+    } else if (p->alg != SW1D_CLASSIC) {
         int W = q->node_width > 0 ? q->node_width : tpb;
         if (p->eq == EQ_KS && p->alg == SW1D_ALT && q->node_width <= 0) W = 128;
         const int pmin = min_points_per_lane(p->eq), pmax = max_points_per_lane(p->eq, p->f64);
@@ -639,6 +692,8 @@ int sw1d_advance(sw1d_plan* p, int64_t nsteps)
     if (p->alg == SW1D_CLASSIC) {
         rc = SW1D_OK;
         for (int64_t s = 0; s < nsteps && rc == SW1D_OK; ++s) rc = classic_step(p);
+    } else if (p->alg == SW1D_TRAPEZOID) {
+        rc = trapezoid_advance(p, nsteps);
     } else {
         rc = swept_advance(p, nsteps);
     }
@@ -736,6 +791,215 @@ int sw1d_schedule(int32_t kind, double dt, double tf_arg, double freq, int32_t t
         steps[i] = rows[i].steps;
         times[i] = rows[i].time;
     }
+    return SW1D_OK;
+}
+
+}  // extern "C"
+
+// ============================================================ batches ===
+struct sw1d_batch {
+    bool f64 = true;
+    size_t S = 8;
+    int64_t count = 0, total = 0;
+    std::vector<int64_t> offset, n;
+    std::vector<char> jobs_host;  // BatchJob<T>[count], queue order
+    void* jobs = nullptr;
+    int* next = nullptr;
+    void* a = nullptr;
+    void* b = nullptr;
+    int32_t grid = 0;
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int64_t graph_steps = -1;
+};
+
+namespace {
+
+template <typename T>
+void build_jobs(sw1d_batch* B, const int32_t* eq, const double* dt, const double* dx,
+                const int32_t* tile_points)
+{
+    std::vector<int64_t> order((size_t)B->count);
+    for (int64_t i = 0; i < B->count; ++i) order[i] = i;
+    // largest problems first: a greedy longest-processing-time queue
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int64_t x, int64_t y) { return B->n[x] > B->n[y]; });
+    B->jobs_host.assign(sizeof(BatchJob<T>) * (size_t)B->count, 0);
+    BatchJob<T>* J = reinterpret_cast<BatchJob<T>*>(B->jobs_host.data());
+    for (int64_t k = 0; k < B->count; ++k) {
+        const int64_t i = order[k];
+        BatchJob<T>& j = J[k];
+        j.offset = B->offset[i];
+        j.n = (int32_t)B->n[i];
+        j.eq = eq[i];
+        int tp = tile_points ? tile_points[i] : 0;
+        if (tp <= 0) tp = B->n[i] >= 16384 ? 512 : (B->n[i] >= 4096 ? 256 : 128);
+        j.po = tp / 64;
+        if (eq[i] == EQ_HEAT) {
+            heat_consts<T>(dt[i], 8.418e-5, dx[i], j.c[0], j.c[1]);
+        } else {
+            const KsCoef<T> k2 = ks_consts<T>(dt[i], dx[i]);
+            j.c[0] = k2.dx4x; j.c[1] = k2.dx2; j.c[2] = k2.dx4; j.c[3] = k2.dt; j.c[4] = k2.dth;
+        }
+    }
+}
+
+void batch_release(sw1d_batch* B)
+{
+    if (!B) return;
+    if (B->stream) (void)hipStreamSynchronize(B->stream);
+    if (B->exec) (void)hipGraphExecDestroy(B->exec);
+    if (B->graph) (void)hipGraphDestroy(B->graph);
+    if (B->jobs) (void)hipFree(B->jobs);
+    if (B->next) (void)hipFree(B->next);
+    if (B->a) (void)hipFree(B->a);
+    if (B->b) (void)hipFree(B->b);
+    if (B->stream) (void)hipStreamDestroy(B->stream);
+    delete B;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sw1d_batch_create(sw1d_batch** out, int32_t precision, int64_t count, const int32_t* eq,
+                      const int64_t* nx, const double* dt, const double* dx,
+                      const int32_t* tile_points)
+{
+    if (!out || !eq || !nx || !dt || !dx) return fail(SW1D_EINVAL, "null argument");
+    *out = nullptr;
+    if (count <= 0 || count > (1 << 30)) return fail(SW1D_EINVAL, "bad problem count %lld", (long long)count);
+    if (precision != SW1D_F64 && precision != SW1D_F32) return fail(SW1D_EINVAL, "unknown precision");
+    sw1d_batch* B = new sw1d_batch();
+    B->f64 = precision == SW1D_F64;
+    B->S = B->f64 ? 8 : 4;
+    B->count = count;
+    B->offset.resize((size_t)count);
+    B->n.resize((size_t)count);
+    for (int64_t i = 0; i < count; ++i) {
+        if (eq[i] != SW1D_HEAT && eq[i] != SW1D_KS) {
+            delete B;
+            return fail(SW1D_EINVAL, "problem %lld: batches hold periodic heat or KS", (long long)i);
+        }
+        if (nx[i] < 128 || nx[i] > (int64_t)1 << 30 || !(dt[i] > 0) || !(dx[i] > 0)) {
+            delete B;
+            return fail(SW1D_EINVAL, "problem %lld: nx >= 128, dt > 0, dx > 0 required", (long long)i);
+        }
+        const int tp = tile_points ? tile_points[i] : 0;
+        if (tp != 0 && !batch_tile_supported(tp / 64)) {
+            delete B;
+            return fail(SW1D_EINVAL, "problem %lld: tile_points must be 128, 256 or 512", (long long)i);
+        }
+        B->offset[i] = B->total;
+        B->n[i] = nx[i];
+        B->total += nx[i];
+    }
+    if (B->f64) build_jobs<double>(B, eq, dt, dx, tile_points);
+    else build_jobs<float>(B, eq, dt, dx, tile_points);
+
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+        delete B;
+        return fail(SW1D_EDEVICE, "no HIP device");
+    }
+    const int64_t g = (int64_t)cus * batch_blocks_per_cu(B->f64);
+    B->grid = (int32_t)std::min<int64_t>(g, count);
+    const size_t bytes = (size_t)B->total * B->S;
+    if (hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc(&B->a, bytes) != hipSuccess || hipMalloc(&B->b, bytes) != hipSuccess ||
+        hipMalloc(&B->jobs, B->jobs_host.size()) != hipSuccess ||
+        hipMalloc((void**)&B->next, sizeof(int)) != hipSuccess) {
+        batch_release(B);
+        return fail(SW1D_ENOMEM, "batch device allocation of %zu bytes failed", 2 * bytes);
+    }
+    hipError_t e = hipMemcpy(B->jobs, B->jobs_host.data(), B->jobs_host.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        batch_release(B);
+        return fail(SW1D_EDEVICE, "job table upload: %s", hipGetErrorString(e));
+    }
+    *out = B;
+    return SW1D_OK;
+}
+
+int sw1d_batch_destroy(sw1d_batch* B)
+{
+    batch_release(B);
+    return SW1D_OK;
+}
+
+int sw1d_batch_set_state(sw1d_batch* B, const void* host)
+{
+    if (!B || !host) return fail(SW1D_EINVAL, "null argument");
+    HIP_OR_FAIL(hipMemcpyAsync(B->a, host, (size_t)B->total * B->S, hipMemcpyHostToDevice, B->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(B->stream));
+    return SW1D_OK;
+}
+
+int sw1d_batch_get_state(sw1d_batch* B, void* host)
+{
+    if (!B || !host) return fail(SW1D_EINVAL, "null argument");
+    HIP_OR_FAIL(hipMemcpyAsync(host, B->a, (size_t)B->total * B->S, hipMemcpyDeviceToHost, B->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(B->stream));
+    return SW1D_OK;
+}
+
+int sw1d_batch_advance(sw1d_batch* B, int64_t nsteps)
+{
+    if (!B) return fail(SW1D_EINVAL, "null batch");
+    if (nsteps < 0) return fail(SW1D_EINVAL, "nsteps must be >= 0");
+    if (nsteps == 0) return SW1D_OK;
+    if (B->graph_steps != nsteps) {
+        if (B->exec) { (void)hipGraphExecDestroy(B->exec); B->exec = nullptr; }
+        if (B->graph) { (void)hipGraphDestroy(B->graph); B->graph = nullptr; }
+        BatchLaunch L;
+        L.jobs = B->jobs;
+        L.njobs = (int32_t)B->count;
+        L.next = B->next;
+        L.a = B->a;
+        L.b = B->b;
+        L.nsteps = nsteps;
+        L.grid = B->grid;
+        HIP_OR_FAIL(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
+        const hipError_t le = launch_batch(L, B->f64, B->stream);
+        const hipError_t ce = hipStreamEndCapture(B->stream, &B->graph);
+        if (le != hipSuccess) return fail(SW1D_EDEVICE, "batch launch capture: %s", hipGetErrorString(le));
+        if (ce != hipSuccess) return fail(SW1D_EDEVICE, "hipStreamEndCapture: %s", hipGetErrorString(ce));
+        HIP_OR_FAIL(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
+        B->graph_steps = nsteps;
+    }
+    HIP_OR_FAIL(hipGraphLaunch(B->exec, B->stream));
+    return SW1D_OK;
+}
+
+int sw1d_batch_advance_timed(sw1d_batch* B, int64_t nsteps, double* ms)
+{
+    if (!B || !ms) return fail(SW1D_EINVAL, "null argument");
+    hipEvent_t e0, e1;
+    HIP_OR_FAIL(hipEventCreate(&e0));
+    HIP_OR_FAIL(hipEventCreate(&e1));
+    HIP_OR_FAIL(hipEventRecord(e0, B->stream));
+    int rc = sw1d_batch_advance(B, nsteps);
+    if (rc == SW1D_OK) {
+        HIP_OR_FAIL(hipEventRecord(e1, B->stream));
+        HIP_OR_FAIL(hipEventSynchronize(e1));
+        float f = 0;
+        HIP_OR_FAIL(hipEventElapsedTime(&f, e0, e1));
+        *ms = f;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+int sw1d_batch_info(const sw1d_batch* B, int64_t info[4])
+{
+    if (!B || !info) return fail(SW1D_EINVAL, "null argument");
+    info[0] = B->total;
+    info[1] = B->grid;
+    info[2] = B->count;
+    info[3] = (int64_t)(2 * (size_t)B->total * B->S + B->jobs_host.size());
     return SW1D_OK;
 }
 

@@ -105,6 +105,34 @@ SW1D_HD T eu_pdiff(const Q3<T>& a, const Q3<T>& b)
     return (T(2) * b.x * a.x * (b.z - a.z) + (a.y * a.y * b.x - b.y * b.y * a.x));
 }
 
+// One interface between cells B and C (A and Dc are the next cells out), with
+// q0 = pdiff(A,B), q1 = pdiff(B,C), q2 = pdiff(C,Dc).  The two limited states
+// and the flux / dissipation of Euler:326-332 (left interface of a cell) and
+// Euler:336-341 (right interface): the right interface of cell i and the left
+// interface of cell i+1 are the same expressions on the same operands.
+template <typename T>
+SW1D_HD void eu_iface(const Q3<T>& A, const Q3<T>& B, const Q3<T>& C, const Q3<T>& Dc, T q0,
+                      T q1, T q2, const EulerCoef<T>& k, Q3<T>& F, Q3<T>& D)
+{
+    const Q3<T> tL = (!q0 || !q1 || ((q0 < 0) != (q1 < 0))) ? B
+                   : eu_limit(B, C, (A.x * q1 / (C.x * q0)));
+    const Q3<T> tR = (!q1 || !q2 || ((q1 < 0) != (q2 < 0))) ? C
+                   : eu_limit(C, B, (Dc.x * q1 / (B.x * q2)));
+    eu_flux_diss(tL, tR, k, F, D);
+}
+
+// Cell balance in the reference's accumulation order (Euler:331-341):
+// flux = F_left; flux += D_left; flux -= F_right; flux -= D_right.
+template <typename T>
+SW1D_HD Q3<T> eu_balance(const Q3<T>& FL, const Q3<T>& DL, const Q3<T>& FR, const Q3<T>& DR)
+{
+    Q3<T> a = FL;
+    a.x = a.x + DL.x; a.y = a.y + DL.y; a.z = a.z + DL.z;
+    a.x = a.x - FR.x; a.y = a.y - FR.y; a.z = a.z - FR.z;
+    a.x = a.x - DR.x; a.y = a.y - DR.y; a.z = a.z - DR.z;
+    return a;
+}
+
 // Net flux of cell 2 of the five cells s0..s4 (Euler:307-341 / 353-380).
 // flagL / flagR drop the outermost pressure difference (cells 1 and n-2).
 template <typename T>
@@ -116,22 +144,17 @@ SW1D_HD Q3<T> eu_net_flux(const Q3<T>& s0, const Q3<T>& s1, const Q3<T>& s2,
     const T pL = eu_pdiff(s1, s2);
     const T pR = eu_pdiff(s2, s3);
     const T pRR = flagR ? T(0) : eu_pdiff(s3, s4);
-    const bool mid = (!pL || !pR || ((pL < 0) != (pR < 0)));
+    Q3<T> FL, DL, FR, DR;
+    eu_iface(s0, s1, s2, s3, pLL, pL, pR, k, FL, DL);
+    eu_iface(s1, s2, s3, s4, pL, pR, pRR, k, FR, DR);
+    return eu_balance(FL, DL, FR, DR);
+}
 
-    Q3<T> tL = (!pLL || !pL || ((pLL < 0) != (pL < 0))) ? s1
-             : eu_limit(s1, s2, (s0.x * pL / (s2.x * pLL)));
-    Q3<T> tR = mid ? s2 : eu_limit(s2, s1, (s3.x * pL / (s1.x * pR)));
-    Q3<T> acc, F, D;
-    eu_flux_diss(tL, tR, k, acc, D);  // flux = eulerFlux(...)
-    acc.x = acc.x + D.x; acc.y = acc.y + D.y; acc.z = acc.z + D.z;
-
-    tL = mid ? s2 : eu_limit(s2, s3, (s1.x * pR / (s3.x * pL)));
-    tR = (!pRR || !pR || ((pRR < 0) != (pR < 0))) ? s3
-       : eu_limit(s3, s2, (s4.x * pR / (s2.x * pRR)));
-    eu_flux_diss(tL, tR, k, F, D);
-    acc.x = acc.x - F.x; acc.y = acc.y - F.y; acc.z = acc.z - F.z;
-    acc.x = acc.x - D.x; acc.y = acc.y - D.y; acc.z = acc.z - D.z;
-    return acc;
+// Predictor / corrector update of one cell from its net flux.
+template <typename T>
+SW1D_HD Q3<T> eu_apply(const Q3<T>& base, const Q3<T>& f, T c)
+{
+    return Q3<T>{base.x + c * f.x, base.y + c * f.y, base.z + c * f.z};
 }
 
 // Euler:307-350 (eulerStutterStep): Q + (1/4 dt/dx) * flux.

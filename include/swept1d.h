@@ -44,8 +44,10 @@ enum { SW1D_HEAT = 0, SW1D_KS = 1, SW1D_EULER = 2 };
 /* Precision: the -DREAL=double build (Makefile:4) or the float default. */
 enum { SW1D_F64 = 0, SW1D_F32 = 1 };
 /* Algorithm = argv[6] of the reference: 0 classic, 1 shared swept,
- * 2 hybrid CPU+GPU swept (heat, Euler) / register swept (KS). */
-enum { SW1D_CLASSIC = 0, SW1D_SWEPT = 1, SW1D_ALT = 2 };
+ * 2 hybrid CPU+GPU swept (heat, Euler) / register swept (KS).
+ * 3 (library only, no reference counterpart): overlapped trapezoid tiles —
+ * swept up-triangles widened by a redundant halo, any nx, same bits. */
+enum { SW1D_CLASSIC = 0, SW1D_SWEPT = 1, SW1D_ALT = 2, SW1D_TRAPEZOID = 3 };
 /* Boundary conditions.  DEFAULT picks the reference's: heat insulated
  * (mirror, Heat:140-142), KS periodic (KS:199), Euler Dirichlet ghost cells
  * (Euler:405-416).  PERIODIC heat is the synthetic periodic-heat workload. */
@@ -62,6 +64,8 @@ typedef struct sw1d_problem {
     double dt;           /* time step, argv[3] */
     double dx;           /* 0: reference value (heat ds 0.001, KS 0.5, Euler 1/(nx-2)) */
     double alpha;        /* heat diffusivity; 0: reference 8.418e-5 (Heat:56) */
+    int32_t tile_depth;  /* SW1D_TRAPEZOID: stages per launch in units of 32/R (0: 1) */
+    int32_t reserved;    /* must be 0 */
 } sw1d_problem;
 
 typedef struct sw1d_plan sw1d_plan;
@@ -133,6 +137,29 @@ int sw1d_kernel_stats(sw1d_plan* plan, int32_t kind, int64_t* launches, double* 
 int sw1d_schedule(int32_t kind, double dt, double tf_arg, double freq, int32_t tpb,
                   double dt_sched, int64_t* out_steps, double* out_times,
                   int64_t capacity, int64_t* count);
+
+/* Batched solves -------------------------------------------------------------
+ * No reference counterpart (the reference solves one problem per process):
+ * many independent periodic heat / KS problems of any sizes, advanced together
+ * by one persistent kernel launch, replayed from a captured hipGraph.
+ * Problem i: equation[i] (SW1D_HEAT periodic or SW1D_KS), nx[i] >= 128 points,
+ * time step dt[i], grid spacing dx[i] (heat diffusivity 8.418e-5), tile width
+ * tile_points[i] output points per trapezoid tile (128, 256 or 512; 0 picks
+ * by size).  The state is the concatenation of the problems' fields in index
+ * order, REAL per the precision.  Every problem's result is bitwise the
+ * single-problem result. */
+typedef struct sw1d_batch sw1d_batch;
+int sw1d_batch_create(sw1d_batch** batch, int32_t precision, int64_t count,
+                      const int32_t* equation, const int64_t* nx, const double* dt,
+                      const double* dx, const int32_t* tile_points);
+int sw1d_batch_destroy(sw1d_batch* batch);
+int sw1d_batch_set_state(sw1d_batch* batch, const void* host);
+int sw1d_batch_get_state(sw1d_batch* batch, void* host);
+int sw1d_batch_advance(sw1d_batch* batch, int64_t nsteps);
+int sw1d_batch_advance_timed(sw1d_batch* batch, int64_t nsteps, double* elapsed_ms);
+/* info[0] total points, info[1] persistent workgroups, info[2] problems,
+ * info[3] device bytes */
+int sw1d_batch_info(const sw1d_batch* batch, int64_t info[4]);
 
 #ifdef __cplusplus
 }

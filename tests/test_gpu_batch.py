@@ -1,0 +1,53 @@
+"""Batched persistent solve: every problem of a mixed heat/KS batch of uneven
+sizes is bitwise the oracle's classic result (periodic BCs)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def wl():
+    import importlib
+
+    return importlib.import_module("1dsweptcuda_amd.workloads")
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("nsteps", [203, 40])
+def test_batch_matches_oracle(sw, orc, wl, prec, nsteps):
+    count = 40
+    eq, nx, dt, dx = wl.batch_problems(count, seed=prec + 5, lo=7.0, hi=13.0)
+    tps = np.array([(128, 256, 512)[i % 3] for i in range(count)], np.int32)
+    dtype = np.float64 if prec == 0 else np.float32
+    ics = [u.astype(dtype) for u in wl.batch_ics(eq, nx, dx)]
+    print(f"batch prec={prec} steps={nsteps} sizes={nx.min()}..{nx.max()}", flush=True)
+    with sw.Batch(eq, nx, dt, dx, precision=prec, tile_points=tps) as b:
+        b.set_state(np.concatenate(ics))
+        b.advance(nsteps)
+        got = b.split(b.get_state())
+    for i in range(count):
+        if eq[i] == 0:
+            fo, fo2 = orc.heat_consts(dt[i], dtype, ds=dx[i])
+            want = orc.heat_run(ics[i], nsteps, fo, fo2, 1)
+        else:
+            want = orc.ks_run(ics[i], nsteps, orc.ks_consts(dt[i], dtype, dx=dx[i]))
+        assert np.array_equal(got[i], want), (i, int(eq[i]), int(nx[i]), int(tps[i]))
+
+
+def test_batch_replay_and_repeat(sw, orc, wl):
+    """Two advances of k steps (graph replayed) equal one oracle run of 2k."""
+    eq, nx, dt, dx = wl.batch_problems(12, seed=3, lo=8.0, hi=11.0)
+    ics = wl.batch_ics(eq, nx, dx)
+    with sw.Batch(eq, nx, dt, dx) as b:
+        b.set_state(np.concatenate(ics))
+        b.advance(64)
+        b.advance(64)
+        got = b.split(b.get_state())
+    for i in range(12):
+        if eq[i] == 0:
+            fo, fo2 = orc.heat_consts(dt[i], ds=dx[i])
+            want = orc.heat_run(ics[i], 128, fo, fo2, 1)
+        else:
+            want = orc.ks_run(ics[i], 128, orc.ks_consts(dt[i], dx=dx[i]))
+        assert np.array_equal(got[i], want), i

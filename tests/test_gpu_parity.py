@@ -101,3 +101,32 @@ def test_node_widths(sw, orc, eq, W):
         got = p.get_state()
     want = _oracle(orc, eq, u0, nsteps, dt, dtype, 0, n)
     assert np.array_equal(got, want)
+
+
+TRAP_CASES = []
+for eq, bc in ((0, 0), (0, 2), (1, 0), (2, 0)):
+    for prec in (0, 1):
+        for W, depth in ((128, 1), (256, 1), (512, 1), (1024, 2)):
+            if eq == 2 and W > 256:
+                continue
+            TRAP_CASES.append((eq, bc, prec, W, depth))
+
+
+@pytest.mark.parametrize("eq,bc,prec,W,depth", TRAP_CASES)
+def test_trapezoid_parity(sw, orc, eq, bc, prec, W, depth):
+    # periodic problems take any nx (last tile clipped); the others nx = k*W
+    n = 4099 if (eq == 1 or bc == 2) else 4096
+    if W >= 1024:
+        n = 3 * W + (5 if (eq == 1 or bc == 2) else 0)
+    dtype = np.float64 if prec == 0 else np.float32
+    dt = _dt(eq, n, bc)
+    nsteps = 97 if eq != 0 else 301
+    _say(f"trapezoid eq={eq} bc={bc} prec={prec} W={W} depth={depth} n={n} steps={nsteps}")
+    u0 = _ic(orc, eq, n, dtype, bc)
+    with sw.Plan(eq, n, dt, precision=prec, algorithm=3, node_width=W, tile_depth=depth, bc=bc) as p:
+        p.set_state(u0)
+        p.advance(nsteps)
+        got = p.get_state()
+    want = _oracle(orc, eq, u0, nsteps, dt, dtype, bc, n)
+    bad = np.flatnonzero(got != want)
+    assert bad.size == 0, f"{bad.size} mismatches, first at {bad[:8]}"
