@@ -1,0 +1,131 @@
+"""Tile-size autotuner for the batched solver.
+
+The reference picks its threads-per-block by brute force: every run of a tpb
+sweep appends a timing line, and the analysis keeps the fastest configuration
+per problem size (pyAnalysisTools/analysis_help.py:31-64, `QualityRuns`).  The
+same idea here, on the device and per size class: problems are grouped by
+floor(log2 nX); for each class a sample batch is timed with every candidate
+tile width (output points per trapezoid tile), and the class gets the fastest.
+
+`choose` is a pure function of the timing table, so a given set of timings
+always gives the same choice: the minimum time per point-update wins, and
+candidates within `tie` of the minimum resolve to the wider tile (fewer
+redundant halo points).  Classes with no timing take the choice of the
+nearest timed class below, else above.
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+import time
+
+import numpy as np
+
+CANDIDATES = (128, 256, 512, 1024)
+DEFAULT_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned", "batch_tiles.json")
+
+
+def size_class(n: int) -> int:
+    return int(math.floor(math.log2(int(n))))
+
+
+def choose(timings: dict, tie: float = 0.02) -> dict:
+    """timings: {class: {tile_points: seconds per point-update}} -> {class: tile_points}."""
+    out = {}
+    for cls in sorted(timings):
+        row = {int(k): float(v) for k, v in timings[cls].items() if v is not None and v > 0}
+        if not row:
+            continue
+        best = min(row.values())
+        near = [tp for tp, v in row.items() if v <= best * (1.0 + tie)]
+        out[int(cls)] = max(near)
+    return out
+
+
+def tile_points_for(nx, table: dict, default: int = 256) -> np.ndarray:
+    """Per-problem tile width from a {class: tile_points} table."""
+    classes = sorted(int(c) for c in table)
+    res = np.empty(len(nx), np.int32)
+    for i, n in enumerate(nx):
+        c = size_class(n)
+        if c in table:
+            res[i] = table[c]
+        elif classes:
+            below = [k for k in classes if k <= c]
+            res[i] = table[below[-1]] if below else table[classes[0]]
+        else:
+            res[i] = default
+    return res
+
+
+def tune_batch(sw, eq, nx, dt, dx, ics, precision: int = 0, steps: int = 400, sample: int = 96,
+               candidates=CANDIDATES, reps: int = 2, seed: int = 0):
+    """Time every candidate per size class on the device; return (table, timings)."""
+    rng = np.random.default_rng(seed)
+    eq, nx, dt, dx = map(np.asarray, (eq, nx, dt, dx))
+    timings = {}
+    for cls in sorted({size_class(n) for n in nx}):
+        idx = np.flatnonzero([size_class(n) == cls for n in nx])
+        if idx.size > sample:
+            idx = np.sort(rng.choice(idx, sample, replace=False))
+        pts = float(nx[idx].sum()) * steps
+        packed = np.concatenate([ics[i] for i in idx])
+        row = {}
+        for tp in candidates:
+            if tp > 2 * int(nx[idx].min()):
+                continue
+            with sw.Batch(eq[idx], nx[idx], dt[idx], dx[idx], precision=precision,
+                          tile_points=np.full(idx.size, tp, np.int32)) as b:
+                b.set_state(packed)
+                b.advance(steps)  # instantiate the graph, warm the caches
+                ms = min(b.advance_timed(steps) for _ in range(reps))
+            row[tp] = ms * 1e-3 / pts
+        timings[cls] = row
+    return choose(timings), timings
+
+
+def save(table: dict, timings: dict, path: str = DEFAULT_TABLE, meta: dict | None = None) -> None:
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    with open(path, "w") as f:
+        json.dump({"table": {str(k): v for k, v in table.items()},
+                   "seconds_per_point_update": {str(k): {str(t): s for t, s in r.items()}
+                                                for k, r in timings.items()},
+                   "meta": meta or {}, "written": time.strftime("%Y-%m-%d %H:%M:%S")}, f, indent=1)
+
+
+def load(path: str = DEFAULT_TABLE) -> dict:
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return {int(k): int(v) for k, v in json.load(f)["table"].items()}
+
+
+def main(argv=None):
+    """Tune the batched workload's tile table on the current GPU.
+
+    python 1dsweptcuda_amd/autotune.py [--out PATH] [--steps N] [--precision 0|1]
+    """
+    import argparse
+    import importlib
+    import sys
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=DEFAULT_TABLE)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--precision", type=int, default=0)
+    ap.add_argument("--count", type=int, default=4096)
+    a = ap.parse_args(argv)
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    sw = importlib.import_module("1dsweptcuda_amd")
+    wl = importlib.import_module("1dsweptcuda_amd.workloads")
+    eq, nx, dt, dx = wl.batch_problems(a.count, seed=2024)
+    ics = wl.batch_ics(eq, nx, dx)
+    table, timings = tune_batch(sw, eq, nx, dt, dx, ics, precision=a.precision, steps=a.steps)
+    save(table, timings, a.out, meta={"precision": a.precision, "steps": a.steps,
+                                      "workload": "batch_problems(%d, seed=2024)" % a.count})
+    print(json.dumps({"table": table}))
+
+
+if __name__ == "__main__":
+    main()

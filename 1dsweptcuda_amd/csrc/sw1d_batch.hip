@@ -92,11 +92,13 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch(const BatchJob<T>* __re
         if (J.eq == EQ_HEAT) {
             if (J.po == 2) batch_problem<T, EQ_HEAT, 2>(J, a, b, nsteps, wave, lane);
             else if (J.po == 4) batch_problem<T, EQ_HEAT, 4>(J, a, b, nsteps, wave, lane);
-            else batch_problem<T, EQ_HEAT, 8>(J, a, b, nsteps, wave, lane);
+            else if (J.po == 8) batch_problem<T, EQ_HEAT, 8>(J, a, b, nsteps, wave, lane);
+            else batch_problem<T, EQ_HEAT, 16>(J, a, b, nsteps, wave, lane);
         } else {
             if (J.po == 2) batch_problem<T, EQ_KS, 2>(J, a, b, nsteps, wave, lane);
             else if (J.po == 4) batch_problem<T, EQ_KS, 4>(J, a, b, nsteps, wave, lane);
-            else batch_problem<T, EQ_KS, 8>(J, a, b, nsteps, wave, lane);
+            else if (J.po == 8) batch_problem<T, EQ_KS, 8>(J, a, b, nsteps, wave, lane);
+            else batch_problem<T, EQ_KS, 16>(J, a, b, nsteps, wave, lane);
         }
         // the result ends in b after an odd number of ping-pongs: copy it home
         if (swaps & 1) {

@@ -77,7 +77,7 @@ struct BatchLaunch {
 hipError_t launch_batch(const BatchLaunch& L, bool f64, hipStream_t s);
 int batch_blocks_per_cu(bool f64);
 constexpr int kBatchThreads = 256;
-inline bool batch_tile_supported(int po) { return po == 2 || po == 4 || po == 8; }
+inline bool batch_tile_supported(int po) { return po == 2 || po == 4 || po == 8 || po == 16; }
 hipError_t launch_trapezoid(const TrapLaunch& L, bool f64, const Coefs& c, hipStream_t s);
 bool trapezoid_supported(int eq, int PO, int SB);
 inline int64_t trapezoid_steps(int eq, int SB) { return eq == EQ_HEAT ? 32 * SB : 8 * SB; }

@@ -834,7 +834,7 @@ void build_jobs(sw1d_batch* B, const int32_t* eq, const double* dt, const double
         j.n = (int32_t)B->n[i];
         j.eq = eq[i];
         int tp = tile_points ? tile_points[i] : 0;
-        if (tp <= 0) tp = B->n[i] >= 16384 ? 512 : (B->n[i] >= 4096 ? 256 : 128);
+        if (tp <= 0) tp = 512;  // widest tile that won every size class of the tuned workload
         j.po = tp / 64;
         if (eq[i] == EQ_HEAT) {
             heat_consts<T>(dt[i], 8.418e-5, dx[i], j.c[0], j.c[1]);
@@ -889,7 +889,7 @@ int sw1d_batch_create(sw1d_batch** out, int32_t precision, int64_t count, const 
         const int tp = tile_points ? tile_points[i] : 0;
         if (tp != 0 && !batch_tile_supported(tp / 64)) {
             delete B;
-            return fail(SW1D_EINVAL, "problem %lld: tile_points must be 128, 256 or 512", (long long)i);
+            return fail(SW1D_EINVAL, "problem %lld: tile_points must be 128, 256, 512 or 1024", (long long)i);
         }
         B->offset[i] = B->total;
         B->n[i] = nx[i];

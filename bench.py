@@ -56,6 +56,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the trapezoid engine and the batched workload")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -134,9 +136,73 @@ def main():
                      "flop_per_launch": flops,
                      "algorithmic_bytes_per_launch": 32 * n},
     }
+    if not args.no_extras:
+        out["trapezoid"] = _trapezoid(sw, n, dt, rng_ic, args)
+        out["batched"] = _batched(sw)
     if not args.no_cpu:
         out["cpu_baseline"] = _cpu_baseline(n, dt, rng_ic, args.cpu_seconds)
     print(json.dumps(out))
+
+
+def _trapezoid(sw, n, dt, u0, args):
+    """Same workload on the overlapped trapezoid engine (algorithm 3)."""
+    res = {}
+    for W, depth in ((512, 1), (1024, 2)):
+        p = sw.Plan(sw.KS, n, dt, algorithm=sw.TRAPEZOID, node_width=W, tile_depth=depth)
+        p.set_state(u0)
+        p.advance(args.warmup)
+        secs = _timed(p, args.steps)
+        p.close()
+        res["W%d_depth%d" % (W, depth)] = n * args.steps / secs
+    best = max(res, key=res.get)
+    return {"value": res[best], "config": best, "all": res, "unit": "point-updates/s"}
+
+
+def _batched(sw, count=4096, steps=5000, loop_sample=512):
+    """4096 uneven periodic heat/KS problems, 5000 steps, one persistent launch
+    per precision, against a per-problem launch loop (trapezoid plans)."""
+    wl = importlib.import_module("1dsweptcuda_amd.workloads")
+    at = importlib.import_module("1dsweptcuda_amd.autotune")
+    eq, nx, dt, dx = wl.batch_problems(count, seed=2024)
+    ics = wl.batch_ics(eq, nx, dx)
+    table = at.load()
+    tps = at.tile_points_for(nx, table) if table else None
+    total = float(nx.sum())
+    res = {"problems": count, "steps": steps, "total_points": int(total),
+           "tile_table": {str(k): v for k, v in table.items()} or "size heuristic"}
+    packed = np.concatenate(ics)
+    for prec, name in ((sw.F64, "f64"), (sw.F32, "f32")):
+        with sw.Batch(eq, nx, dt, dx, precision=prec, tile_points=tps) as b:
+            b.set_state(packed)
+            b.advance(steps)  # captures the graph
+            b.set_state(packed)
+            t0 = time.perf_counter()
+            b.advance(steps)
+            b.get_state()
+            secs = time.perf_counter() - t0
+        res[name] = {"value": total * steps / secs, "seconds": secs}
+    # per-problem launch loop on an evenly spaced sample, FP64
+    idx = np.arange(0, count, max(count // loop_sample, 1))
+    plans = []
+    for i in idx:
+        p = sw.Plan(int(eq[i]), int(nx[i]), float(dt[i]), algorithm=sw.TRAPEZOID, node_width=256,
+                    bc=sw.BC_PERIODIC, dx=float(dx[i]))
+        p.set_state(ics[i])
+        plans.append(p)
+    for p in plans:
+        p.synchronize()
+    t0 = time.perf_counter()
+    for p in plans:
+        p.advance(steps)
+    for p in plans:
+        p.synchronize()
+    secs = time.perf_counter() - t0
+    for p in plans:
+        p.close()
+    loop = float(nx[idx].sum()) * steps / secs
+    res["per_problem_loop_f64"] = {"value": loop, "sample_problems": int(idx.size)}
+    res["speedup_vs_loop_f64"] = res["f64"]["value"] / loop
+    return res
 
 
 def _ks_ic(n, seed):

@@ -144,8 +144,8 @@ int sw1d_schedule(int32_t kind, double dt, double tf_arg, double freq, int32_t t
  * by one persistent kernel launch, replayed from a captured hipGraph.
  * Problem i: equation[i] (SW1D_HEAT periodic or SW1D_KS), nx[i] >= 128 points,
  * time step dt[i], grid spacing dx[i] (heat diffusivity 8.418e-5), tile width
- * tile_points[i] output points per trapezoid tile (128, 256 or 512; 0 picks
- * by size).  The state is the concatenation of the problems' fields in index
+ * tile_points[i] output points per trapezoid tile (128, 256, 512 or 1024;
+ * 0 picks 512).  The state is the concatenation of the problems' fields in index
  * order, REAL per the precision.  Every problem's result is bitwise the
  * single-problem result. */
 typedef struct sw1d_batch sw1d_batch;
