@@ -299,3 +299,123 @@ class Batch:
 
     def split(self, packed):
         return [packed[self.offsets[i]:self.offsets[i + 1]] for i in range(len(self.nx))]
+
+
+class Surrogate:
+    """The learned KS surrogate (sw1d_nn_*): periodic 1D CNN, BF16 MFMA on the GPU."""
+
+    def __init__(self, nx: int = 1024, max_batch: int = 64, seed: int = 0):
+        L = lib()
+        P = C.c_void_p
+        L.sw1d_nn_last_error.restype = C.c_char_p
+        L.sw1d_nn_create.argtypes = [C.POINTER(P), C.c_int32, C.c_int32, C.c_uint64]
+        L.sw1d_nn_destroy.argtypes = [P]
+        L.sw1d_nn_set_scales.argtypes = [P, C.c_double, C.c_double]
+        L.sw1d_nn_get_params.argtypes = [P, P]
+        L.sw1d_nn_set_params.argtypes = [P, P]
+        L.sw1d_nn_forward.argtypes = [P, P, C.c_int32, P]
+        L.sw1d_nn_gradient.argtypes = [P, P, P, C.c_int32, C.POINTER(C.c_double), P]
+        L.sw1d_nn_train.argtypes = [P, P, P, C.c_int64, C.c_int32, C.c_int64, C.c_float, C.c_uint64,
+                                    C.POINTER(C.c_double)]
+        L.sw1d_surrogate_advance.argtypes = [P, P, C.c_int64, C.c_int32, C.c_int32, C.c_double,
+                                             C.POINTER(C.c_double)]
+        L.sw1d_nn_dataset.argtypes = [P, P, C.c_int32, C.c_int32, C.POINTER(C.c_int64),
+                                      C.POINTER(C.c_double)]
+        L.sw1d_nn_fit.argtypes = [P, C.c_int64, C.c_int32, C.c_int64, C.c_float, C.c_uint64,
+                                  C.POINTER(C.c_double)]
+        L.sw1d_nn_eval.argtypes = [P, C.c_int64, C.c_int64, C.POINTER(C.c_double)]
+        self.nx, self.max_batch = nx, max_batch
+        self._h = C.c_void_p()
+        self._chk(L.sw1d_nn_create(C.byref(self._h), nx, max_batch, seed), "sw1d_nn_create")
+
+    @staticmethod
+    def param_count() -> int:
+        return lib().sw1d_nn_param_count()
+
+    @staticmethod
+    def device_check() -> bool:
+        return lib().sw1d_nn_device_check() == 1
+
+    def _chk(self, rc, where):
+        if rc < 0:
+            raise SweptError(rc, where, lib().sw1d_nn_last_error().decode())
+        return rc
+
+    def close(self):
+        if self._h:
+            lib().sw1d_nn_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_scales(self, in_scale: float, out_scale: float) -> None:
+        self._chk(lib().sw1d_nn_set_scales(self._h, in_scale, out_scale), "sw1d_nn_set_scales")
+
+    def get_params(self) -> np.ndarray:
+        p = np.empty(self.param_count(), np.float32)
+        self._chk(lib().sw1d_nn_get_params(self._h, p.ctypes.data_as(C.c_void_p)), "sw1d_nn_get_params")
+        return p
+
+    def set_params(self, p) -> None:
+        a = np.ascontiguousarray(p, np.float32)
+        self._chk(lib().sw1d_nn_set_params(self._h, a.ctypes.data_as(C.c_void_p)), "sw1d_nn_set_params")
+
+    def forward(self, x) -> np.ndarray:
+        a = np.ascontiguousarray(x, np.float32).reshape(-1, self.nx)
+        y = np.empty_like(a)
+        self._chk(lib().sw1d_nn_forward(self._h, a.ctypes.data_as(C.c_void_p), a.shape[0],
+                                        y.ctypes.data_as(C.c_void_p)), "sw1d_nn_forward")
+        return y
+
+    def gradient(self, x, t):
+        a = np.ascontiguousarray(x, np.float32).reshape(-1, self.nx)
+        b = np.ascontiguousarray(t, np.float32).reshape(-1, self.nx)
+        g = np.empty(self.param_count(), np.float32)
+        loss = C.c_double(0)
+        self._chk(lib().sw1d_nn_gradient(self._h, a.ctypes.data_as(C.c_void_p), b.ctypes.data_as(C.c_void_p),
+                                         a.shape[0], C.byref(loss), g.ctypes.data_as(C.c_void_p)),
+                  "sw1d_nn_gradient")
+        return loss.value, g
+
+    def train(self, d_inputs: int, d_targets: int, nsamples: int, batch: int, iters: int, lr: float,
+              seed: int = 0) -> float:
+        loss = C.c_double(0)
+        self._chk(lib().sw1d_nn_train(self._h, C.c_void_p(d_inputs), C.c_void_p(d_targets), nsamples, batch,
+                                      iters, lr, seed, C.byref(loss)), "sw1d_nn_train")
+        return loss.value
+
+    def dataset(self, batch: "Batch", nsnap: int, k_steps: int = 20):
+        """Snapshots of a KS batch every k_steps as the training set; returns
+        (number of samples, (in_scale, out_scale))."""
+        n = C.c_int64(0)
+        sc = (C.c_double * 2)()
+        self._chk(lib().sw1d_nn_dataset(self._h, batch._h, nsnap, k_steps, C.byref(n), sc), "sw1d_nn_dataset")
+        return n.value, (sc[0], sc[1])
+
+    def fit(self, iters: int, batch: int = 32, lr: float = 1e-3, nfit: int = 0, seed: int = 0) -> float:
+        loss = C.c_double(0)
+        self._chk(lib().sw1d_nn_fit(self._h, nfit, batch, iters, lr, seed, C.byref(loss)), "sw1d_nn_fit")
+        return loss.value
+
+    def eval(self, first: int, count: int) -> float:
+        e = C.c_double(0)
+        self._chk(lib().sw1d_nn_eval(self._h, first, count, C.byref(e)), "sw1d_nn_eval")
+        return e.value
+
+    def advance(self, batch: "Batch", surrogate_steps: int, k_steps: int = 20, judge_every: int = 10,
+                tol: float = 1e-2) -> dict:
+        st = (C.c_double * 4)()
+        self._chk(lib().sw1d_surrogate_advance(self._h, batch._h, surrogate_steps, k_steps, judge_every, tol,
+                                               st), "sw1d_surrogate_advance")
+        return {"network_steps": int(st[0]), "judged": int(st[1]), "fallbacks": int(st[2]),
+                "max_judged_error": st[3]}

@@ -72,6 +72,19 @@ struct BatchLaunch {
     int32_t grid;      // persistent workgroups
 };
 
+}  // namespace sw1d
+
+struct sw1d_batch;
+namespace sw1d {
+// Device view of a batch for the surrogate integrator (sw1d_nn.hip).
+struct BatchView {
+    void* state;            // packed state (problem i at offset i*n when uniform)
+    int64_t count, n;       // problems; their common size (0 if sizes differ)
+    bool f64, all_ks;
+    hipStream_t stream;
+};
+bool batch_view(sw1d_batch* b, BatchView& v);
+
 // Launchers (sw1d_kernels.hip, sw1d_trapezoid.hip, sw1d_batch.hip).  All
 // return a hipError_t.
 hipError_t launch_batch(const BatchLaunch& L, bool f64, hipStream_t s);

@@ -1,0 +1,1039 @@
+// Learned KS surrogate: a periodic 1D convolutional network advancing the KS
+// field by K steps at a time, trained and run on gfx950.
+//
+// Network (6 layers, kernel 5, periodic padding, GELU between layers):
+//   L1   1 -> 64 channels   FP32 VALU (5 MACs per output)
+//   L2-5 64 -> 64 channels  BF16 MFMA v_mfma_f32_16x16x32_bf16, FP32 accumulate
+//   L6   64 -> 1 channel    FP32 VALU over BF16 activations
+// output du = L6(...); the integrator adds it to u (residual form).
+//
+// Activations are channel-last [B][X][64] BF16; pre-activations Z are kept in
+// FP32 for the GELU derivative.  A 64->64 layer is a GEMM per 64-position
+// tile: M = positions, N = output channels, K = 5 taps x 64 input channels,
+// staged in LDS in the operand layouts of the MFMA (8 consecutive K values per
+// lane, 16-byte LDS reads).  The weight gradient is a GEMM over positions
+// (K = positions) with both operands staged transposed; each workgroup writes
+// its partial sum and a second kernel adds the partials in a fixed order, so
+// gradients are deterministic.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/swept1d.h"
+
+namespace sw1dnn {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int C = 64;       // hidden channels
+constexpr int TAPS = 5;     // kernel width
+constexpr int TILE = 64;    // positions per workgroup
+constexpr int NT = 256;     // threads per workgroup
+constexpr int ROWP = C + 8; // padded LDS row (bf16 elements)
+
+// ---------------------------------------------------------------- bf16 ---
+// Round to nearest even (NaN kept quiet), the same rule the host oracle uses.
+__host__ __device__ inline uint16_t f2bf_bits(float f)
+{
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__host__ __device__ inline float bf_bits2f(uint16_t h)
+{
+    const uint32_t u = (uint32_t)h << 16;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+__device__ inline __bf16 f2bf(float f) { return __builtin_bit_cast(__bf16, f2bf_bits(f)); }
+__device__ inline float bf2f(__bf16 h) { return bf_bits2f(__builtin_bit_cast(uint16_t, h)); }
+
+// GELU(z) = z/2 (1 + erf(z/sqrt 2)) and its derivative.
+__device__ inline float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+__device__ inline float gelu_d(float z)
+{
+    return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) +
+           z * 0.39894228040143268f * expf(-0.5f * z * z);
+}
+
+__device__ inline int wrapx(int x, int X) { return x < 0 ? x + X : (x >= X ? x - X : x); }
+
+// ------------------------------------------------------ MFMA layout check ---
+// Assumed v_mfma_f32_16x16x32_bf16 operand layout (lane l = 0..63):
+//   A (16x32): lane holds A[l%16][8*(l/16) + j], j = 0..7
+//   B (32x16): lane holds B[8*(l/16) + j][l%16]
+//   D (16x16): lane holds D[4*(l/16) + j][l%16], j = 0..3
+// Checked against a plain product of small integers (exact in BF16/FP32).
+__global__ void k_mfma_check(int* ok)
+{
+    const int l = threadIdx.x;
+    bf16x8 a, b;
+    for (int j = 0; j < 8; ++j) {
+        const int k = 8 * (l / 16) + j;
+        a[j] = f2bf((float)(((l % 16) * 3 + k * 5) % 7 - 3));
+        b[j] = f2bf((float)((k * 2 + (l % 16) * 7) % 5 - 2));
+    }
+    f32x4 d = {0.f, 0.f, 0.f, 0.f};
+    d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, d, 0, 0, 0);
+    for (int j = 0; j < 4; ++j) {
+        const int m = 4 * (l / 16) + j, n = l % 16;
+        float want = 0.f;
+        for (int k = 0; k < 32; ++k)
+            want += (float)((m * 3 + k * 5) % 7 - 3) * (float)((k * 2 + n * 7) % 5 - 2);
+        if (d[j] != want) atomicAnd(ok, 0);
+    }
+}
+
+// ------------------------------------------------------ L1: 1 -> 64, VALU ---
+// Z1[b][x][c] = b1[c] + sum_t w1[c][t] u[b][x+t-2];  A1 = bf16(gelu(Z1))
+__global__ __launch_bounds__(NT) void k_l1_fwd(const float* __restrict__ u, const float* __restrict__ w1,
+                                               const float* __restrict__ b1, float* __restrict__ Z,
+                                               __bf16* __restrict__ A, int X, int B)
+{
+    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;  // (b, x, c)
+    if (i >= (int64_t)B * X * C) return;
+    const int c = (int)(i % C);
+    const int64_t bx = i / C;
+    const int x = (int)(bx % X), b = (int)(bx / X);
+    const float* ub = u + (int64_t)b * X;
+    float z = b1[c];
+    for (int t = 0; t < TAPS; ++t) z += w1[c * TAPS + t] * ub[wrapx(x + t - 2, X)];
+    Z[i] = z;
+    A[i] = f2bf(gelu(z));
+}
+
+// ----------------------------------------------- L2-L5: 64 -> 64, MFMA ---
+// Z[b][x][co] = bias[co] + sum_t sum_ci A[b][x+t-2][ci] W[t][co][ci]
+__global__ __launch_bounds__(NT) void k_mid_fwd(const __bf16* __restrict__ Ain, const __bf16* __restrict__ Wf,
+                                                const float* __restrict__ bias, float* __restrict__ Z,
+                                                __bf16* __restrict__ Aout, int X)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 sA[(TILE + 4) * ROWP];
+    __shared__ __attribute__((aligned(16))) __bf16 sW[TAPS * C * ROWP];
+    const int b = blockIdx.y, x0 = blockIdx.x * TILE;
+    const __bf16* Ab = Ain + (int64_t)b * X * C;
+    // stage rows x0-2 .. x0+TILE+1 (16-byte chunks: 8 channels)
+    for (int q = threadIdx.x; q < (TILE + 4) * (C / 8); q += NT) {
+        const int r = q / (C / 8), ch = (q % (C / 8)) * 8;
+        const int x = wrapx(x0 - 2 + r, X);
+        *reinterpret_cast<uint4*>(&sA[r * ROWP + ch]) =
+            *reinterpret_cast<const uint4*>(&Ab[(int64_t)x * C + ch]);
+    }
+    for (int q = threadIdx.x; q < TAPS * C * (C / 8); q += NT) {
+        const int r = q / (C / 8), ch = (q % (C / 8)) * 8;  // r = t*C + co
+        *reinterpret_cast<uint4*>(&sW[r * ROWP + ch]) = *reinterpret_cast<const uint4*>(&Wf[r * C + ch]);
+    }
+    __syncthreads();
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int m0 = 16 * w, li = l % 16, lk = 8 * (l / 16);
+    f32x4 acc[4];
+    for (int nb = 0; nb < 4; ++nb) acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int s = 0; s < TAPS * 2; ++s) {
+        const int t = s >> 1, cb = (s & 1) * 32;
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sA[(m0 + li + t) * ROWP + cb + lk]);
+        for (int nb = 0; nb < 4; ++nb) {
+            const bf16x8 bv = *reinterpret_cast<const bf16x8*>(&sW[(t * C + 16 * nb + li) * ROWP + cb + lk]);
+            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc[nb], 0, 0, 0);
+        }
+    }
+    for (int nb = 0; nb < 4; ++nb) {
+        const int co = 16 * nb + li;
+        for (int j = 0; j < 4; ++j) {
+            const int x = x0 + m0 + 4 * (l / 16) + j;
+            const float z = acc[nb][j] + bias[co];
+            const int64_t o = ((int64_t)b * X + x) * C + co;
+            Z[o] = z;
+            Aout[o] = f2bf(gelu(z));
+        }
+    }
+}
+
+// dA[b][y][ci] = sum_t sum_co dZ[b][y-t+2][co] W[t][co][ci]   (W as Wb[t][ci][co])
+// then dZprev = bf16(dA * gelu'(Zprev)).
+__global__ __launch_bounds__(NT) void k_mid_bwd_data(const __bf16* __restrict__ dZ, const __bf16* __restrict__ Wb,
+                                                     const float* __restrict__ Zprev,
+                                                     __bf16* __restrict__ dZprev, int X)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 sD[(TILE + 4) * ROWP];
+    __shared__ __attribute__((aligned(16))) __bf16 sW[TAPS * C * ROWP];
+    const int b = blockIdx.y, y0 = blockIdx.x * TILE;
+    const __bf16* Db = dZ + (int64_t)b * X * C;
+    for (int q = threadIdx.x; q < (TILE + 4) * (C / 8); q += NT) {
+        const int r = q / (C / 8), ch = (q % (C / 8)) * 8;
+        const int x = wrapx(y0 - 2 + r, X);
+        *reinterpret_cast<uint4*>(&sD[r * ROWP + ch]) =
+            *reinterpret_cast<const uint4*>(&Db[(int64_t)x * C + ch]);
+    }
+    for (int q = threadIdx.x; q < TAPS * C * (C / 8); q += NT) {
+        const int r = q / (C / 8), ch = (q % (C / 8)) * 8;  // r = t*C + ci
+        *reinterpret_cast<uint4*>(&sW[r * ROWP + ch]) = *reinterpret_cast<const uint4*>(&Wb[r * C + ch]);
+    }
+    __syncthreads();
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int m0 = 16 * w, li = l % 16, lk = 8 * (l / 16);
+    f32x4 acc[4];
+    for (int nb = 0; nb < 4; ++nb) acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int s = 0; s < TAPS * 2; ++s) {
+        const int t = s >> 1, cb = (s & 1) * 32;
+        // row y = y0+m0+li reads dZ at y-t+2, staged row (y-t+2)-(y0-2)
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sD[(m0 + li - t + 4) * ROWP + cb + lk]);
+        for (int nb = 0; nb < 4; ++nb) {
+            const bf16x8 bv = *reinterpret_cast<const bf16x8*>(&sW[(t * C + 16 * nb + li) * ROWP + cb + lk]);
+            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc[nb], 0, 0, 0);
+        }
+    }
+    for (int nb = 0; nb < 4; ++nb) {
+        const int ci = 16 * nb + li;
+        for (int j = 0; j < 4; ++j) {
+            const int y = y0 + m0 + 4 * (l / 16) + j;
+            const int64_t o = ((int64_t)b * X + y) * C + ci;
+            dZprev[o] = f2bf(acc[nb][j] * gelu_d(Zprev[o]));
+        }
+    }
+}
+
+// Partial weight / bias gradients of one 64-position tile:
+// dW[t][co][ci] += sum_x dZ[x][co] A[x+t-2][ci];  db[co] += sum_x dZ[x][co]
+// Both operands are staged transposed (positions contiguous), one shifted
+// copy of A per tap, so every MFMA operand is a 16-byte LDS read.
+__global__ __launch_bounds__(NT) void k_mid_bwd_weight(const __bf16* __restrict__ dZ, const __bf16* __restrict__ Ain,
+                                                       float* __restrict__ part, int X)
+{
+    constexpr int XP = TILE + 8;  // padded position row
+    __shared__ __attribute__((aligned(16))) __bf16 sDT[C * XP];
+    __shared__ __attribute__((aligned(16))) __bf16 sAT[TAPS * C * XP];
+    const int b = blockIdx.y, x0 = blockIdx.x * TILE;
+    const int64_t wg = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+    const __bf16* Db = dZ + (int64_t)b * X * C;
+    const __bf16* Ab = Ain + (int64_t)b * X * C;
+    for (int q = threadIdx.x; q < TILE * C; q += NT) {
+        const int x = q / C, c = q % C;
+        sDT[c * XP + x] = Db[(int64_t)(x0 + x) * C + c];
+        for (int t = 0; t < TAPS; ++t)
+            sAT[(t * C + c) * XP + x] = Ab[(int64_t)wrapx(x0 + x + t - 2, X) * C + c];
+    }
+    __syncthreads();
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int li = l % 16, lk = 8 * (l / 16);
+    const int co0 = 16 * w;  // wave w: output rows co0..co0+15, all ci, all taps
+    float* pw = part + wg * (TAPS * C * C + C);
+    for (int t = 0; t < TAPS; ++t) {
+        f32x4 acc[4];
+        for (int nb = 0; nb < 4; ++nb) acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int ks = 0; ks < TILE; ks += 32) {
+            const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sDT[(co0 + li) * XP + ks + lk]);
+            for (int nb = 0; nb < 4; ++nb) {
+                const bf16x8 bv =
+                    *reinterpret_cast<const bf16x8*>(&sAT[(t * C + 16 * nb + li) * XP + ks + lk]);
+                acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc[nb], 0, 0, 0);
+            }
+        }
+        for (int nb = 0; nb < 4; ++nb)
+            for (int j = 0; j < 4; ++j) {
+                const int co = co0 + 4 * (l / 16) + j, ci = 16 * nb + li;
+                pw[(t * C + co) * C + ci] = acc[nb][j];
+            }
+    }
+    if (threadIdx.x < C) {
+        float s = 0.f;
+        for (int x = 0; x < TILE; ++x) s += bf2f(sDT[threadIdx.x * XP + x]);
+        pw[TAPS * C * C + threadIdx.x] = s;
+    }
+}
+
+// out[i] = sum_{w < nparts} part[w][i], fixed order.
+__global__ __launch_bounds__(NT) void k_reduce_parts(const float* __restrict__ part, int64_t nparts,
+                                                     int64_t len, float* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (i >= len) return;
+    float s = 0.f;
+    for (int64_t w = 0; w < nparts; ++w) s += part[w * len + i];
+    out[i] = s;
+}
+
+// ------------------------------------------------------ L6: 64 -> 1, VALU ---
+// du[b][x] = b6 + sum_t sum_ci w6[t][ci] A5[b][x+t-2][ci]
+__global__ __launch_bounds__(NT) void k_l6_fwd(const __bf16* __restrict__ A, const float* __restrict__ w6,
+                                               const float* __restrict__ b6, float* __restrict__ du, int X, int B)
+{
+    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (i >= (int64_t)B * X) return;
+    const int x = (int)(i % X), b = (int)(i / X);
+    const __bf16* Ab = A + (int64_t)b * X * C;
+    float s = b6[0];
+    for (int t = 0; t < TAPS; ++t) {
+        const __bf16* row = Ab + (int64_t)wrapx(x + t - 2, X) * C;
+        for (int c = 0; c < C; ++c) s += w6[t * C + c] * bf2f(row[c]);
+    }
+    du[i] = s;
+}
+
+// loss = mean (du - target)^2 over the batch (partials per block) and
+// g = d loss / d du = 2 (du - target) / N.
+__global__ __launch_bounds__(NT) void k_mse(const float* __restrict__ du, const float* __restrict__ tgt,
+                                            float* __restrict__ g, float* __restrict__ part, int64_t N)
+{
+    __shared__ float red[NT];
+    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+    float e = 0.f;
+    if (i < N) {
+        const float d = du[i] - tgt[i];
+        g[i] = 2.f * d / (float)N;
+        e = d * d;
+    }
+    red[threadIdx.x] = e;
+    __syncthreads();
+    for (int s = NT / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// dA5[b][y][ci] = sum_t g[b][y-t+2] w6[t][ci];  dZ5 = bf16(dA5 * gelu'(Z5))
+__global__ __launch_bounds__(NT) void k_l6_bwd_data(const float* __restrict__ g, const float* __restrict__ w6,
+                                                    const float* __restrict__ Z5, __bf16* __restrict__ dZ5,
+                                                    int X, int B)
+{
+    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (i >= (int64_t)B * X * C) return;
+    const int ci = (int)(i % C);
+    const int64_t bx = i / C;
+    const int y = (int)(bx % X), b = (int)(bx / X);
+    float s = 0.f;
+    for (int t = 0; t < TAPS; ++t) s += g[(int64_t)b * X + wrapx(y - t + 2, X)] * w6[t * C + ci];
+    dZ5[i] = f2bf(s * gelu_d(Z5[i]));
+}
+
+// Per-tile partials: dw6[t][ci] = sum_x g[x] A5[x+t-2][ci]; db6 = sum_x g[x]
+__global__ __launch_bounds__(NT) void k_l6_bwd_weight(const float* __restrict__ g, const __bf16* __restrict__ A,
+                                                      float* __restrict__ part, int X)
+{
+    const int b = blockIdx.y, x0 = blockIdx.x * TILE;
+    const int64_t wg = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+    float* pw = part + wg * (TAPS * C + 1);
+    for (int q = threadIdx.x; q < TAPS * C; q += NT) {
+        const int t = q / C, ci = q % C;
+        float s = 0.f;
+        for (int x = x0; x < x0 + TILE; ++x)
+            s += g[(int64_t)b * X + x] * bf2f(A[((int64_t)b * X + wrapx(x + t - 2, X)) * C + ci]);
+        pw[q] = s;
+    }
+    if (threadIdx.x == 0) {
+        float s = 0.f;
+        for (int x = x0; x < x0 + TILE; ++x) s += g[(int64_t)b * X + x];
+        pw[TAPS * C] = s;
+    }
+}
+
+// Per-tile partials: dw1[c][t] = sum_x dZ1[x][c] u[x+t-2]; db1[c] = sum_x dZ1[x][c]
+__global__ __launch_bounds__(NT) void k_l1_bwd_weight(const __bf16* __restrict__ dZ1, const float* __restrict__ u,
+                                                      float* __restrict__ part, int X)
+{
+    const int b = blockIdx.y, x0 = blockIdx.x * TILE;
+    const int64_t wg = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+    float* pw = part + wg * (C * TAPS + C);
+    for (int q = threadIdx.x; q < C * (TAPS + 1); q += NT) {
+        const int c = q / (TAPS + 1), t = q % (TAPS + 1);
+        float s = 0.f;
+        for (int x = x0; x < x0 + TILE; ++x) {
+            const float d = bf2f(dZ1[((int64_t)b * X + x) * C + c]);
+            s += t < TAPS ? d * u[(int64_t)b * X + wrapx(x + t - 2, X)] : d;
+        }
+        if (t < TAPS) pw[c * TAPS + t] = s;
+        else pw[C * TAPS + c] = s;
+    }
+}
+
+// -------------------------------------------------------------- optimiser ---
+__global__ __launch_bounds__(NT) void k_adam(float* __restrict__ p, const float* __restrict__ g,
+                                             float* __restrict__ m, float* __restrict__ v, int64_t n,
+                                             float lr, float b1, float b2, float eps, float c1, float c2)
+{
+    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// BF16 copies of a 64->64 layer's weights in the forward ([t][co][ci]) and
+// backward ([t][ci][co]) layouts.
+__global__ __launch_bounds__(NT) void k_pack_w(const float* __restrict__ w, __bf16* __restrict__ wf,
+                                               __bf16* __restrict__ wb)
+{
+    const int i = blockIdx.x * NT + threadIdx.x;
+    if (i >= TAPS * C * C) return;
+    const int t = i / (C * C), co = (i / C) % C, ci = i % C;
+    const __bf16 h = f2bf(w[i]);
+    wf[i] = h;
+    wb[(t * C + ci) * C + co] = h;
+}
+
+// u (FP64 state) -> normalised FP32 network input; and u += scale * du.
+__global__ __launch_bounds__(NT) void k_to_input(const double* __restrict__ u, float* __restrict__ in,
+                                                 int64_t n, float inv_scale)
+{
+    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (i < n) in[i] = (float)u[i] * inv_scale;
+}
+__global__ __launch_bounds__(NT) void k_apply_du(double* __restrict__ u, const float* __restrict__ du,
+                                                 int64_t n, double scale)
+{
+    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (i < n) u[i] += scale * (double)du[i];
+}
+
+// Relative L2 distance ||a - b|| / ||b|| partial sums per block: (sum (a-b)^2, sum b^2).
+__global__ __launch_bounds__(NT) void k_l2_parts(const double* __restrict__ a, const double* __restrict__ b,
+                                                 int64_t n, double* __restrict__ part)
+{
+    __shared__ double r0[NT], r1[NT];
+    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+    double d = 0, s = 0;
+    if (i < n) {
+        d = (a[i] - b[i]) * (a[i] - b[i]);
+        s = b[i] * b[i];
+    }
+    r0[threadIdx.x] = d;
+    r1[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = NT / 2; k > 0; k >>= 1) {
+        if (threadIdx.x < k) {
+            r0[threadIdx.x] += r0[threadIdx.x + k];
+            r1[threadIdx.x] += r1[threadIdx.x + k];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = r0[0];
+        part[2 * blockIdx.x + 1] = r1[0];
+    }
+}
+
+}  // namespace sw1dnn
+
+// ================================================================== host ===
+namespace sw1dnn {
+
+// parameter vector (FP32), in this order
+constexpr int64_t P_W1 = 0, P_B1 = P_W1 + C * TAPS;
+constexpr int64_t P_MID = P_B1 + C;                  // 4 x (W[t][co][ci], b[co])
+constexpr int64_t MID_SZ = TAPS * C * C + C;
+constexpr int64_t P_W6 = P_MID + 4 * MID_SZ, P_B6 = P_W6 + TAPS * C;
+constexpr int64_t NPARAM = P_B6 + 1;
+
+inline unsigned nblk(int64_t n) { return (unsigned)((n + NT - 1) / NT); }
+
+uint64_t splitmix(uint64_t& s)
+{
+    uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+double normal(uint64_t& s)
+{
+    const double u1 = 1.0 - (double)(splitmix(s) >> 11) * (1.0 / 9007199254740992.0);
+    const double u2 = (double)(splitmix(s) >> 11) * (1.0 / 9007199254740992.0);
+    return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+}  // namespace sw1dnn
+
+using namespace sw1dnn;
+
+namespace {
+thread_local std::string g_nn_err;
+int nn_fail(int code, const char* msg, hipError_t e = hipSuccess)
+{
+    g_nn_err = msg;
+    if (e != hipSuccess) g_nn_err += std::string(": ") + hipGetErrorString(e);
+    return code;
+}
+#define NN_HIP(expr)                                          \
+    do {                                                      \
+        hipError_t e_ = (expr);                               \
+        if (e_ != hipSuccess) return nn_fail(SW1D_EDEVICE, #expr, e_); \
+    } while (0)
+int g_mfma_ok = 0;
+}  // namespace
+
+struct sw1d_nn {
+    int X = 1024, Bmax = 0;
+    double in_scale = 1.0, out_scale = 1.0;
+    hipStream_t stream = nullptr;
+    float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;  // NPARAM each
+    int64_t adam_t = 0;
+    __bf16 *wf[4] = {}, *wb[4] = {};                                // packed mid weights
+    float* x = nullptr;     // [B][X] input
+    float* du = nullptr;    // [B][X] output
+    float* tgt = nullptr;   // [B][X] target
+    float* gdu = nullptr;   // [B][X] d loss / d du
+    float* Z[5] = {};       // [B][X][C] pre-activations of L1..L5
+    __bf16* A[5] = {};      // [B][X][C] activations of L1..L5
+    __bf16 *dZa = nullptr, *dZb = nullptr;  // [B][X][C] gradients
+    float* part = nullptr;  // per-tile partial gradients
+    float* lossp = nullptr;
+    int64_t part_len = 0;
+    // surrogate integration scratch
+    double* usave = nullptr;
+    double* unn = nullptr;
+    double* l2p = nullptr;
+    int64_t int_len = 0;
+};
+
+namespace {
+
+int pack_weights(sw1d_nn* N)
+{
+    for (int l = 0; l < 4; ++l) {
+        hipLaunchKernelGGL(k_pack_w, dim3(nblk(TAPS * C * C)), dim3(NT), 0, N->stream,
+                           N->p + P_MID + l * MID_SZ, N->wf[l], N->wb[l]);
+    }
+    NN_HIP(hipGetLastError());
+    return SW1D_OK;
+}
+
+// forward of B samples already in N->x
+int nn_forward(sw1d_nn* N, int B)
+{
+    const int X = N->X;
+    const int64_t nfeat = (int64_t)B * X * C;
+    hipLaunchKernelGGL(k_l1_fwd, dim3(nblk(nfeat)), dim3(NT), 0, N->stream, N->x, N->p + P_W1,
+                       N->p + P_B1, N->Z[0], N->A[0], X, B);
+    for (int l = 0; l < 4; ++l) {
+        const float* bias = N->p + P_MID + l * MID_SZ + TAPS * C * C;
+        hipLaunchKernelGGL(k_mid_fwd, dim3(X / TILE, B), dim3(NT), 0, N->stream, N->A[l], N->wf[l], bias,
+                           N->Z[l + 1], N->A[l + 1], X);
+    }
+    hipLaunchKernelGGL(k_l6_fwd, dim3(nblk((int64_t)B * X)), dim3(NT), 0, N->stream, N->A[4],
+                       N->p + P_W6, N->p + P_B6, N->du, X, B);
+    NN_HIP(hipGetLastError());
+    return SW1D_OK;
+}
+
+// loss (mean square against N->tgt) and the full gradient into N->g
+int nn_backward(sw1d_nn* N, int B, double* loss)
+{
+    const int X = N->X;
+    const int64_t nout = (int64_t)B * X, nfeat = nout * C;
+    const unsigned lb = nblk(nout);
+    hipLaunchKernelGGL(k_mse, dim3(lb), dim3(NT), 0, N->stream, N->du, N->tgt, N->gdu, N->lossp, nout);
+    const dim3 tiles(X / TILE, B);
+    const int64_t ntiles = (int64_t)(X / TILE) * B;
+    // L6
+    hipLaunchKernelGGL(k_l6_bwd_weight, tiles, dim3(NT), 0, N->stream, N->gdu, N->A[4], N->part, X);
+    hipLaunchKernelGGL(k_reduce_parts, dim3(nblk(TAPS * C + 1)), dim3(NT), 0, N->stream, N->part, ntiles,
+                       (int64_t)(TAPS * C + 1), N->g + P_W6);
+    hipLaunchKernelGGL(k_l6_bwd_data, dim3(nblk(nfeat)), dim3(NT), 0, N->stream, N->gdu, N->p + P_W6,
+                       N->Z[4], N->dZa, X, B);
+    // L5 .. L2
+    __bf16* dz = N->dZa;
+    __bf16* dzp = N->dZb;
+    for (int l = 3; l >= 0; --l) {
+        hipLaunchKernelGGL(k_mid_bwd_weight, tiles, dim3(NT), 0, N->stream, dz, N->A[l], N->part, X);
+        hipLaunchKernelGGL(k_reduce_parts, dim3(nblk(MID_SZ)), dim3(NT), 0, N->stream, N->part, ntiles,
+                           MID_SZ, N->g + P_MID + l * MID_SZ);
+        hipLaunchKernelGGL(k_mid_bwd_data, tiles, dim3(NT), 0, N->stream, dz, N->wb[l], N->Z[l], dzp, X);
+        __bf16* t = dz; dz = dzp; dzp = t;
+    }
+    // L1 (dz now holds dZ1)
+    hipLaunchKernelGGL(k_l1_bwd_weight, tiles, dim3(NT), 0, N->stream, dz, N->x, N->part, X);
+    hipLaunchKernelGGL(k_reduce_parts, dim3(nblk(C * TAPS + C)), dim3(NT), 0, N->stream, N->part, ntiles,
+                       (int64_t)(C * TAPS + C), N->g + P_W1);
+    NN_HIP(hipGetLastError());
+    if (loss) {
+        std::vector<float> lp(lb);
+        NN_HIP(hipMemcpyAsync(lp.data(), N->lossp, lb * sizeof(float), hipMemcpyDeviceToHost, N->stream));
+        NN_HIP(hipStreamSynchronize(N->stream));
+        double s = 0;
+        for (float f : lp) s += f;
+        *loss = s / (double)nout;
+    }
+    return SW1D_OK;
+}
+
+int adam(sw1d_nn* N, float lr, float b1, float b2, float eps)
+{
+    N->adam_t += 1;
+    const float c1 = 1.f - powf(b1, (float)N->adam_t), c2 = 1.f - powf(b2, (float)N->adam_t);
+    hipLaunchKernelGGL(k_adam, dim3(nblk(NPARAM)), dim3(NT), 0, N->stream, N->p, N->g, N->m, N->v,
+                       NPARAM, lr, b1, b2, eps, c1, c2);
+    NN_HIP(hipGetLastError());
+    return pack_weights(N);
+}
+
+void nn_release(sw1d_nn* N)
+{
+    if (!N) return;
+    if (N->stream) (void)hipStreamSynchronize(N->stream);
+    void* ptrs[] = {N->p, N->g, N->m, N->v, N->x, N->du, N->tgt, N->gdu, N->dZa, N->dZb, N->part,
+                    N->lossp, N->usave, N->unn, N->l2p};
+    for (void* q : ptrs)
+        if (q) (void)hipFree(q);
+    for (int l = 0; l < 5; ++l) {
+        if (N->Z[l]) (void)hipFree(N->Z[l]);
+        if (N->A[l]) (void)hipFree(N->A[l]);
+    }
+    for (int l = 0; l < 4; ++l) {
+        if (N->wf[l]) (void)hipFree(N->wf[l]);
+        if (N->wb[l]) (void)hipFree(N->wb[l]);
+    }
+    if (N->stream) (void)hipStreamDestroy(N->stream);
+    delete N;
+}
+
+template <typename T>
+hipError_t dalloc(T** p, size_t count)
+{
+    return hipMalloc((void**)p, count * sizeof(T));
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sw1d_nn_last_error(void) { return g_nn_err.c_str(); }
+
+int sw1d_nn_param_count(void) { return (int)NPARAM; }
+
+int sw1d_nn_device_check(void)
+{
+    int* d = nullptr;
+    NN_HIP(hipMalloc(&d, sizeof(int)));
+    const int one = 1;
+    int ok = 0;
+    hipError_t e = hipMemcpy(d, &one, sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_mfma_check, dim3(1), dim3(64), 0, 0, d);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(&ok, d, sizeof(int), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return nn_fail(SW1D_EDEVICE, "MFMA layout check", e);
+    g_mfma_ok = ok ? 1 : -1;
+    return ok;
+}
+
+int sw1d_nn_create(sw1d_nn** out, int32_t nx, int32_t max_batch, uint64_t seed)
+{
+    if (!out) return nn_fail(SW1D_EINVAL, "null argument");
+    *out = nullptr;
+    if (nx < TILE || nx % TILE != 0 || max_batch < 1)
+        return nn_fail(SW1D_EINVAL, "nx must be a positive multiple of 64 and max_batch >= 1");
+    if (g_mfma_ok == 0 && sw1d_nn_device_check() < 0) return SW1D_EDEVICE;
+    if (g_mfma_ok != 1) return nn_fail(SW1D_EDEVICE, "MFMA operand layout check failed on this device");
+    sw1d_nn* N = new sw1d_nn();
+    N->X = nx;
+    N->Bmax = max_batch;
+    const int64_t nout = (int64_t)nx * max_batch, nfeat = nout * C;
+    const int64_t ntiles = (int64_t)(nx / TILE) * max_batch;
+    N->part_len = ntiles * MID_SZ;
+    bool ok = hipStreamCreateWithFlags(&N->stream, hipStreamNonBlocking) == hipSuccess &&
+              dalloc(&N->p, NPARAM) == hipSuccess && dalloc(&N->g, NPARAM) == hipSuccess &&
+              dalloc(&N->m, NPARAM) == hipSuccess && dalloc(&N->v, NPARAM) == hipSuccess &&
+              dalloc(&N->x, nout) == hipSuccess && dalloc(&N->du, nout) == hipSuccess &&
+              dalloc(&N->tgt, nout) == hipSuccess && dalloc(&N->gdu, nout) == hipSuccess &&
+              dalloc(&N->dZa, nfeat) == hipSuccess && dalloc(&N->dZb, nfeat) == hipSuccess &&
+              dalloc(&N->part, N->part_len) == hipSuccess &&
+              dalloc(&N->lossp, (int64_t)nblk(nout)) == hipSuccess;
+    for (int l = 0; ok && l < 5; ++l)
+        ok = dalloc(&N->Z[l], nfeat) == hipSuccess && dalloc(&N->A[l], nfeat) == hipSuccess;
+    for (int l = 0; ok && l < 4; ++l)
+        ok = dalloc(&N->wf[l], TAPS * C * C) == hipSuccess && dalloc(&N->wb[l], TAPS * C * C) == hipSuccess;
+    if (!ok) {
+        nn_release(N);
+        return nn_fail(SW1D_ENOMEM, "surrogate device allocation failed");
+    }
+    // He-style initialisation (std sqrt(2/fan_in)); the output layer starts at
+    // zero so the untrained network predicts du = 0.
+    std::vector<float> h(NPARAM, 0.f);
+    uint64_t s = seed;
+    for (int64_t i = 0; i < C * TAPS; ++i) h[P_W1 + i] = (float)(normal(s) * sqrt(2.0 / TAPS));
+    for (int l = 0; l < 4; ++l)
+        for (int64_t i = 0; i < TAPS * C * C; ++i)
+            h[P_MID + l * MID_SZ + i] = (float)(normal(s) * sqrt(2.0 / (TAPS * C)));
+    hipError_t e = hipMemcpy(N->p, h.data(), NPARAM * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(N->m, 0, NPARAM * sizeof(float));
+    if (e == hipSuccess) e = hipMemset(N->v, 0, NPARAM * sizeof(float));
+    if (e != hipSuccess) {
+        nn_release(N);
+        return nn_fail(SW1D_EDEVICE, "surrogate initialisation", e);
+    }
+    if (pack_weights(N) != SW1D_OK || hipStreamSynchronize(N->stream) != hipSuccess) {
+        nn_release(N);
+        return SW1D_EDEVICE;
+    }
+    *out = N;
+    return SW1D_OK;
+}
+
+void sw1d_nn_release_dataset(sw1d_nn* N);
+
+int sw1d_nn_destroy(sw1d_nn* N)
+{
+    if (N) sw1d_nn_release_dataset(N);
+    nn_release(N);
+    return SW1D_OK;
+}
+
+int sw1d_nn_set_scales(sw1d_nn* N, double in_scale, double out_scale)
+{
+    if (!N || !(in_scale > 0) || !(out_scale > 0)) return nn_fail(SW1D_EINVAL, "scales must be positive");
+    N->in_scale = in_scale;
+    N->out_scale = out_scale;
+    return SW1D_OK;
+}
+
+int sw1d_nn_get_params(sw1d_nn* N, float* host)
+{
+    if (!N || !host) return nn_fail(SW1D_EINVAL, "null argument");
+    NN_HIP(hipMemcpyAsync(host, N->p, NPARAM * sizeof(float), hipMemcpyDeviceToHost, N->stream));
+    NN_HIP(hipStreamSynchronize(N->stream));
+    return SW1D_OK;
+}
+
+int sw1d_nn_set_params(sw1d_nn* N, const float* host)
+{
+    if (!N || !host) return nn_fail(SW1D_EINVAL, "null argument");
+    NN_HIP(hipMemcpyAsync(N->p, host, NPARAM * sizeof(float), hipMemcpyHostToDevice, N->stream));
+    int rc = pack_weights(N);
+    if (rc) return rc;
+    NN_HIP(hipStreamSynchronize(N->stream));
+    return SW1D_OK;
+}
+
+// y = net(x) for `batch` samples of nx values (normalised units).
+int sw1d_nn_forward(sw1d_nn* N, const float* x_host, int32_t batch, float* y_host)
+{
+    if (!N || !x_host || !y_host) return nn_fail(SW1D_EINVAL, "null argument");
+    if (batch < 1 || batch > N->Bmax) return nn_fail(SW1D_EINVAL, "batch out of range");
+    const size_t bytes = (size_t)batch * N->X * sizeof(float);
+    NN_HIP(hipMemcpyAsync(N->x, x_host, bytes, hipMemcpyHostToDevice, N->stream));
+    int rc = nn_forward(N, batch);
+    if (rc) return rc;
+    NN_HIP(hipMemcpyAsync(y_host, N->du, bytes, hipMemcpyDeviceToHost, N->stream));
+    NN_HIP(hipStreamSynchronize(N->stream));
+    return SW1D_OK;
+}
+
+// Mean-square loss against t and its gradient with respect to every parameter.
+int sw1d_nn_gradient(sw1d_nn* N, const float* x_host, const float* t_host, int32_t batch, double* loss,
+                     float* grad_host)
+{
+    if (!N || !x_host || !t_host || !loss) return nn_fail(SW1D_EINVAL, "null argument");
+    if (batch < 1 || batch > N->Bmax) return nn_fail(SW1D_EINVAL, "batch out of range");
+    const size_t bytes = (size_t)batch * N->X * sizeof(float);
+    NN_HIP(hipMemcpyAsync(N->x, x_host, bytes, hipMemcpyHostToDevice, N->stream));
+    NN_HIP(hipMemcpyAsync(N->tgt, t_host, bytes, hipMemcpyHostToDevice, N->stream));
+    int rc = nn_forward(N, batch);
+    if (!rc) rc = nn_backward(N, batch, loss);
+    if (rc) return rc;
+    if (grad_host) {
+        NN_HIP(hipMemcpyAsync(grad_host, N->g, NPARAM * sizeof(float), hipMemcpyDeviceToHost, N->stream));
+        NN_HIP(hipStreamSynchronize(N->stream));
+    }
+    return SW1D_OK;
+}
+
+// Adam on device-resident samples: inputs and targets are [nsamples][nx]
+// FP32 device arrays (normalised units).  Mini-batches are drawn with a seeded
+// generator on the host and gathered on the device.
+int sw1d_nn_train(sw1d_nn* N, const float* d_inputs, const float* d_targets, int64_t nsamples,
+                  int32_t batch, int64_t iters, float lr, uint64_t seed, double* last_loss)
+{
+    if (!N || !d_inputs || !d_targets) return nn_fail(SW1D_EINVAL, "null argument");
+    if (batch < 1 || batch > N->Bmax || nsamples < 1 || iters < 0)
+        return nn_fail(SW1D_EINVAL, "bad batch / sample count");
+    const size_t row = (size_t)N->X * sizeof(float);
+    uint64_t s = seed;
+    double loss = 0;
+    for (int64_t it = 0; it < iters; ++it) {
+        for (int b = 0; b < batch; ++b) {
+            const int64_t k = (int64_t)(splitmix(s) % (uint64_t)nsamples);
+            NN_HIP(hipMemcpyAsync((char*)N->x + b * row, (const char*)d_inputs + k * row, row,
+                                  hipMemcpyDeviceToDevice, N->stream));
+            NN_HIP(hipMemcpyAsync((char*)N->tgt + b * row, (const char*)d_targets + k * row, row,
+                                  hipMemcpyDeviceToDevice, N->stream));
+        }
+        int rc = nn_forward(N, batch);
+        if (!rc) rc = nn_backward(N, batch, (it == iters - 1 || it % 50 == 0) ? &loss : nullptr);
+        if (!rc) rc = adam(N, lr, 0.9f, 0.999f, 1e-8f);
+        if (rc) return rc;
+    }
+    NN_HIP(hipStreamSynchronize(N->stream));
+    if (last_loss) *last_loss = loss;
+    return SW1D_OK;
+}
+
+}  // extern "C"
+
+// ===================================================== surrogate integrator ===
+#include "sw1d_internal.hpp"
+
+extern "C" int sw1d_surrogate_advance(sw1d_nn* N, sw1d_batch* batch, int64_t surrogate_steps,
+                                      int32_t k_steps, int32_t judge_every, double tol, double* stats)
+{
+    sw1d::BatchView bv;
+    if (!N || !sw1d::batch_view(batch, bv)) return nn_fail(SW1D_EINVAL, "null argument");
+    if (!bv.f64 || !bv.all_ks || bv.n != N->X)
+        return nn_fail(SW1D_EINVAL, "surrogate batches hold FP64 KS problems of the network's nx");
+    if (bv.count > N->Bmax) return nn_fail(SW1D_EINVAL, "batch larger than the network's max_batch");
+    if (k_steps < 1 || surrogate_steps < 0) return nn_fail(SW1D_EINVAL, "bad step counts");
+    const int64_t B = bv.count, X = N->X, len = B * X;
+    const unsigned blocks = nblk(len);
+    if (N->int_len < len) {
+        if (N->usave) (void)hipFree(N->usave);
+        if (N->unn) (void)hipFree(N->unn);
+        if (N->l2p) (void)hipFree(N->l2p);
+        N->usave = N->unn = N->l2p = nullptr;
+        if (dalloc(&N->usave, len) != hipSuccess || dalloc(&N->unn, len) != hipSuccess ||
+            dalloc(&N->l2p, 2 * (int64_t)blocks) != hipSuccess)
+            return nn_fail(SW1D_ENOMEM, "integrator scratch");
+        N->int_len = len;
+    }
+    double* u = static_cast<double*>(bv.state);
+    const size_t bytes = (size_t)len * sizeof(double);
+    int64_t judged = 0, fallbacks = 0;
+    double max_err = 0.0;
+    std::vector<double> parts(2 * (size_t)blocks);
+    NN_HIP(hipStreamSynchronize(bv.stream));
+    for (int64_t i = 1; i <= surrogate_steps; ++i) {
+        const bool judge = judge_every > 0 && i % judge_every == 0;
+        if (judge) NN_HIP(hipMemcpyAsync(N->usave, u, bytes, hipMemcpyDeviceToDevice, N->stream));
+        hipLaunchKernelGGL(k_to_input, dim3(blocks), dim3(NT), 0, N->stream, u, N->x, len,
+                           (float)(1.0 / N->in_scale));
+        int rc = nn_forward(N, (int)B);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_apply_du, dim3(blocks), dim3(NT), 0, N->stream, u, N->du, len, N->out_scale);
+        NN_HIP(hipGetLastError());
+        if (!judge) continue;
+        // plain solver from the same states: u <- solver^K(usave)
+        ++judged;
+        NN_HIP(hipMemcpyAsync(N->unn, u, bytes, hipMemcpyDeviceToDevice, N->stream));
+        NN_HIP(hipMemcpyAsync(u, N->usave, bytes, hipMemcpyDeviceToDevice, N->stream));
+        NN_HIP(hipStreamSynchronize(N->stream));
+        rc = sw1d_batch_advance(batch, k_steps);
+        if (rc) return nn_fail(rc, "plain solver step of the judge");
+        NN_HIP(hipStreamSynchronize(bv.stream));
+        hipLaunchKernelGGL(k_l2_parts, dim3(blocks), dim3(NT), 0, N->stream, N->unn, u, len, N->l2p);
+        NN_HIP(hipMemcpyAsync(parts.data(), N->l2p, parts.size() * sizeof(double), hipMemcpyDeviceToHost,
+                              N->stream));
+        NN_HIP(hipStreamSynchronize(N->stream));
+        const int64_t per = (X + NT - 1) / NT;  // blocks per problem (X is a multiple of NT)
+        for (int64_t p = 0; p < B; ++p) {
+            double d = 0, s = 0;
+            for (int64_t k = p * per; k < (p + 1) * per; ++k) {
+                d += parts[2 * k];
+                s += parts[2 * k + 1];
+            }
+            const double err = s > 0 ? sqrt(d / s) : sqrt(d);
+            max_err = err > max_err ? err : max_err;
+            if (err <= tol) {  // keep the network's state for this problem
+                NN_HIP(hipMemcpyAsync(u + p * X, N->unn + p * X, X * sizeof(double),
+                                      hipMemcpyDeviceToDevice, N->stream));
+            } else {
+                ++fallbacks;   // the plain solver's state stays
+            }
+        }
+    }
+    NN_HIP(hipStreamSynchronize(N->stream));
+    if (stats) {
+        stats[0] = (double)surrogate_steps;
+        stats[1] = (double)judged;
+        stats[2] = (double)fallbacks;
+        stats[3] = max_err;
+    }
+    return SW1D_OK;
+}
+
+// ============================================================ training data ===
+namespace sw1dnn {
+// x = u_s / in_scale, t = (u_{s+1} - u_s) / out_scale for one snapshot pair
+__global__ __launch_bounds__(NT) void k_make_pairs(const double* __restrict__ u0, const double* __restrict__ u1,
+                                                   float* __restrict__ x, float* __restrict__ t, int64_t n,
+                                                   double inv_in, double inv_out)
+{
+    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (i >= n) return;
+    x[i] = (float)(u0[i] * inv_in);
+    t[i] = (float)((u1[i] - u0[i]) * inv_out);
+}
+// partial sums of a^2 and (b-a)^2
+__global__ __launch_bounds__(NT) void k_sq_parts(const double* __restrict__ a, const double* __restrict__ b,
+                                                 int64_t n, double* __restrict__ part)
+{
+    __shared__ double r0[NT], r1[NT];
+    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+    double s0 = 0, s1 = 0;
+    if (i < n) {
+        s0 = a[i] * a[i];
+        s1 = (b[i] - a[i]) * (b[i] - a[i]);
+    }
+    r0[threadIdx.x] = s0;
+    r1[threadIdx.x] = s1;
+    __syncthreads();
+    for (int k = NT / 2; k > 0; k >>= 1) {
+        if (threadIdx.x < k) {
+            r0[threadIdx.x] += r0[threadIdx.x + k];
+            r1[threadIdx.x] += r1[threadIdx.x + k];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = r0[0];
+        part[2 * blockIdx.x + 1] = r1[0];
+    }
+}
+}  // namespace sw1dnn
+
+struct NnData {
+    float* x = nullptr;
+    float* t = nullptr;
+    int64_t nsamples = 0;
+};
+
+namespace {
+std::vector<std::pair<sw1d_nn*, NnData>> g_data;  // dataset per network
+NnData* data_of(sw1d_nn* N)
+{
+    for (auto& e : g_data)
+        if (e.first == N) return &e.second;
+    g_data.push_back({N, NnData{}});
+    return &g_data.back().second;
+}
+void free_data(sw1d_nn* N)
+{
+    for (size_t i = 0; i < g_data.size(); ++i)
+        if (g_data[i].first == N) {
+            if (g_data[i].second.x) (void)hipFree(g_data[i].second.x);
+            if (g_data[i].second.t) (void)hipFree(g_data[i].second.t);
+            g_data.erase(g_data.begin() + (long)i);
+            return;
+        }
+}
+}  // namespace
+
+extern "C" {
+
+// Build the training set from an FP64 KS batch: advance it nsnap times by
+// k_steps, keep every snapshot, and store the normalised pairs in the network
+// (in_scale = RMS of u, out_scale = RMS of the K-step change).
+int sw1d_nn_dataset(sw1d_nn* N, sw1d_batch* batch, int32_t nsnap, int32_t k_steps, int64_t* nsamples,
+                    double* scales)
+{
+    sw1d::BatchView bv;
+    if (!N || !sw1d::batch_view(batch, bv) || nsnap < 1 || k_steps < 1)
+        return nn_fail(SW1D_EINVAL, "bad dataset arguments");
+    if (!bv.f64 || !bv.all_ks || bv.n != N->X)
+        return nn_fail(SW1D_EINVAL, "datasets come from FP64 KS batches of the network's nx");
+    const int64_t len = bv.count * bv.n;
+    const size_t bytes = (size_t)len * sizeof(double);
+    double* raw = nullptr;
+    if (hipMalloc((void**)&raw, bytes * (size_t)(nsnap + 1)) != hipSuccess)
+        return nn_fail(SW1D_ENOMEM, "snapshot storage");
+    int rc = SW1D_OK;
+    NN_HIP(hipStreamSynchronize(bv.stream));
+    NN_HIP(hipMemcpy(raw, bv.state, bytes, hipMemcpyDeviceToDevice));
+    for (int s = 1; s <= nsnap && rc == SW1D_OK; ++s) {
+        rc = sw1d_batch_advance(batch, k_steps);
+        if (rc == SW1D_OK && hipStreamSynchronize(bv.stream) != hipSuccess) rc = SW1D_EDEVICE;
+        if (rc == SW1D_OK &&
+            hipMemcpy(raw + (size_t)s * len, bv.state, bytes, hipMemcpyDeviceToDevice) != hipSuccess)
+            rc = SW1D_EDEVICE;
+    }
+    const unsigned blocks = nblk(len);
+    std::vector<double> parts(2 * (size_t)blocks);
+    double* dparts = nullptr;
+    double su = 0, sd = 0;
+    if (rc == SW1D_OK && hipMalloc((void**)&dparts, parts.size() * sizeof(double)) != hipSuccess) rc = SW1D_ENOMEM;
+    for (int s = 0; s < nsnap && rc == SW1D_OK; ++s) {
+        hipLaunchKernelGGL(k_sq_parts, dim3(blocks), dim3(NT), 0, N->stream, raw + (size_t)s * len,
+                           raw + (size_t)(s + 1) * len, len, dparts);
+        if (hipMemcpyAsync(parts.data(), dparts, parts.size() * sizeof(double), hipMemcpyDeviceToHost,
+                           N->stream) != hipSuccess || hipStreamSynchronize(N->stream) != hipSuccess)
+            rc = SW1D_EDEVICE;
+        for (size_t k = 0; k < blocks; ++k) { su += parts[2 * k]; sd += parts[2 * k + 1]; }
+    }
+    NnData* D = data_of(N);
+    if (rc == SW1D_OK) {
+        const double cnt = (double)len * nsnap;
+        N->in_scale = sqrt(su / cnt);
+        N->out_scale = sqrt(sd / cnt);
+        if (D->x) (void)hipFree(D->x);
+        if (D->t) (void)hipFree(D->t);
+        D->x = D->t = nullptr;
+        D->nsamples = (int64_t)nsnap * bv.count;
+        if (hipMalloc((void**)&D->x, (size_t)D->nsamples * N->X * sizeof(float)) != hipSuccess ||
+            hipMalloc((void**)&D->t, (size_t)D->nsamples * N->X * sizeof(float)) != hipSuccess)
+            rc = SW1D_ENOMEM;
+    }
+    for (int s = 0; s < nsnap && rc == SW1D_OK; ++s)
+        hipLaunchKernelGGL(k_make_pairs, dim3(blocks), dim3(NT), 0, N->stream, raw + (size_t)s * len,
+                           raw + (size_t)(s + 1) * len, D->x + (size_t)s * len, D->t + (size_t)s * len, len,
+                           1.0 / N->in_scale, 1.0 / N->out_scale);
+    if (rc == SW1D_OK && (hipGetLastError() != hipSuccess || hipStreamSynchronize(N->stream) != hipSuccess))
+        rc = SW1D_EDEVICE;
+    if (dparts) (void)hipFree(dparts);
+    (void)hipFree(raw);
+    if (rc != SW1D_OK) return nn_fail(rc, "dataset generation failed");
+    if (nsamples) *nsamples = D->nsamples;
+    if (scales) { scales[0] = N->in_scale; scales[1] = N->out_scale; }
+    return SW1D_OK;
+}
+
+// Adam on samples [0, nfit) of the stored dataset.
+int sw1d_nn_fit(sw1d_nn* N, int64_t nfit, int32_t batch, int64_t iters, float lr, uint64_t seed,
+                double* last_loss)
+{
+    NnData* D = N ? data_of(N) : nullptr;
+    if (!D || !D->x) return nn_fail(SW1D_EINVAL, "no dataset (call sw1d_nn_dataset first)");
+    if (nfit <= 0 || nfit > D->nsamples) nfit = D->nsamples;
+    return sw1d_nn_train(N, D->x, D->t, nfit, batch, iters, lr, seed, last_loss);
+}
+
+// Relative L2 error of the predicted K-step change over samples [first, first+count).
+int sw1d_nn_eval(sw1d_nn* N, int64_t first, int64_t count, double* rel_err)
+{
+    NnData* D = N ? data_of(N) : nullptr;
+    if (!D || !D->x || !rel_err) return nn_fail(SW1D_EINVAL, "no dataset or null argument");
+    if (first < 0 || count < 1 || first + count > D->nsamples) return nn_fail(SW1D_EINVAL, "range");
+    const size_t row = (size_t)N->X;
+    double se = 0, st = 0;
+    std::vector<float> y, t;
+    for (int64_t s = first; s < first + count; s += N->Bmax) {
+        const int b = (int)std::min<int64_t>(N->Bmax, first + count - s);
+        NN_HIP(hipMemcpyAsync(N->x, D->x + s * row, b * row * sizeof(float), hipMemcpyDeviceToDevice, N->stream));
+        int rc = nn_forward(N, b);
+        if (rc) return rc;
+        y.resize(b * row);
+        t.resize(b * row);
+        NN_HIP(hipMemcpyAsync(y.data(), N->du, b * row * sizeof(float), hipMemcpyDeviceToHost, N->stream));
+        NN_HIP(hipMemcpyAsync(t.data(), D->t + s * row, b * row * sizeof(float), hipMemcpyDeviceToHost, N->stream));
+        NN_HIP(hipStreamSynchronize(N->stream));
+        for (size_t k = 0; k < y.size(); ++k) {
+            se += (double)(y[k] - t[k]) * (y[k] - t[k]);
+            st += (double)t[k] * t[k];
+        }
+    }
+    *rel_err = sqrt(se / st);
+    return SW1D_OK;
+}
+
+void sw1d_nn_release_dataset(sw1d_nn* N) { free_data(N); }
+
+}  // extern "C"

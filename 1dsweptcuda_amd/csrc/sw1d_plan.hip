@@ -799,6 +799,7 @@ int sw1d_schedule(int32_t kind, double dt, double tf_arg, double freq, int32_t t
 // ============================================================ batches ===
 struct sw1d_batch {
     bool f64 = true;
+    bool all_ks = true;
     size_t S = 8;
     int64_t count = 0, total = 0;
     std::vector<int64_t> offset, n;
@@ -893,6 +894,7 @@ int sw1d_batch_create(sw1d_batch** out, int32_t precision, int64_t count, const 
         }
         B->offset[i] = B->total;
         B->n[i] = nx[i];
+        B->all_ks = B->all_ks && eq[i] == SW1D_KS;
         B->total += nx[i];
     }
     if (B->f64) build_jobs<double>(B, eq, dt, dx, tile_points);
@@ -1004,3 +1006,19 @@ int sw1d_batch_info(const sw1d_batch* B, int64_t info[4])
 }
 
 }  // extern "C"
+
+namespace sw1d {
+bool batch_view(sw1d_batch* B, BatchView& v)
+{
+    if (!B) return false;
+    v.state = B->a;
+    v.count = B->count;
+    v.n = B->n.empty() ? 0 : B->n[0];
+    for (int64_t x : B->n)
+        if (x != v.n) v.n = 0;
+    v.f64 = B->f64;
+    v.all_ks = B->all_ks;
+    v.stream = B->stream;
+    return true;
+}
+}  // namespace sw1d

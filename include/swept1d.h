@@ -161,6 +161,50 @@ int sw1d_batch_advance_timed(sw1d_batch* batch, int64_t nsteps, double* elapsed_
  * info[3] device bytes */
 int sw1d_batch_info(const sw1d_batch* batch, int64_t info[4]);
 
+/* Learned KS surrogate -------------------------------------------------------
+ * No reference counterpart.  A periodic 1D CNN (6 layers, 64 channels, kernel
+ * 5, GELU) predicting the KS change over K solver steps; layers 2-5 run on
+ * BF16 MFMA with FP32 accumulation, forward and backward; trained on the GPU
+ * with Adam.  Inputs and outputs are in normalised units: x = u / in_scale,
+ * y = du / out_scale.  Parameters are one FP32 vector of
+ * sw1d_nn_param_count() values. */
+typedef struct sw1d_nn sw1d_nn;
+const char* sw1d_nn_last_error(void);
+int sw1d_nn_param_count(void);
+/* Checks the MFMA operand layout the kernels assume.  1 = pass. */
+int sw1d_nn_device_check(void);
+int sw1d_nn_create(sw1d_nn** nn, int32_t nx, int32_t max_batch, uint64_t seed);
+int sw1d_nn_destroy(sw1d_nn* nn);
+int sw1d_nn_set_scales(sw1d_nn* nn, double in_scale, double out_scale);
+int sw1d_nn_get_params(sw1d_nn* nn, float* host);
+int sw1d_nn_set_params(sw1d_nn* nn, const float* host);
+int sw1d_nn_forward(sw1d_nn* nn, const float* x_host, int32_t batch, float* y_host);
+/* mean-square loss against t_host and its gradient (grad_host may be NULL) */
+int sw1d_nn_gradient(sw1d_nn* nn, const float* x_host, const float* t_host, int32_t batch,
+                     double* loss, float* grad_host);
+/* Adam over device-resident [nsamples][nx] FP32 inputs / targets */
+int sw1d_nn_train(sw1d_nn* nn, const float* d_inputs, const float* d_targets, int64_t nsamples,
+                  int32_t batch, int64_t iters, float lr, uint64_t seed, double* last_loss);
+/* Training set from an FP64 KS batch of the network's nx: the batch is
+ * advanced nsnap times by k_steps; pairs (u_s / in_scale, (u_{s+1} - u_s) /
+ * out_scale) are kept on the device, in_scale / out_scale = RMS of u and of
+ * the K-step change (written to scales[0..1] and set on the network). */
+int sw1d_nn_dataset(sw1d_nn* nn, sw1d_batch* batch, int32_t nsnap, int32_t k_steps,
+                    int64_t* nsamples, double* scales);
+/* Adam on the first nfit samples of the stored training set */
+int sw1d_nn_fit(sw1d_nn* nn, int64_t nfit, int32_t batch, int64_t iters, float lr, uint64_t seed,
+                double* last_loss);
+/* relative L2 error of the predicted change over stored samples [first, first+count) */
+int sw1d_nn_eval(sw1d_nn* nn, int64_t first, int64_t count, double* rel_err);
+/* Advance every problem of an FP64 KS batch (all of the network's nx) by
+ * surrogate_steps network steps of k_steps solver steps each.  Every
+ * judge_every-th network step the plain solver advances the same states by
+ * k_steps steps; a problem whose relative L2 difference ||u_net - u_plain|| /
+ * ||u_plain|| exceeds tol keeps the plain solver's state (fallback).
+ * stats: {network steps, judged steps, problem fallbacks, max judged error}. */
+int sw1d_surrogate_advance(sw1d_nn* nn, sw1d_batch* batch, int64_t surrogate_steps,
+                           int32_t k_steps, int32_t judge_every, double tol, double* stats);
+
 #ifdef __cplusplus
 }
 #endif
