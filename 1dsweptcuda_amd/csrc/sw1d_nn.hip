@@ -157,6 +157,77 @@ __global__ __launch_bounds__(NT) void k_mid_fwd(const __bf16* __restrict__ Ain, 
     }
 }
 
+// Same product over 256 positions per workgroup: the 40 KB of staged weights
+// serve four times the positions, each wave owns 64 positions x 64 channels
+// (16 accumulator tiles) and reuses every A and B fragment four times.
+// STORE_Z keeps the pre-activations for the backward pass (training only).
+constexpr int TILE_F = 256;
+// LDS rows of 64 BF16 (8 chunks of 16 bytes) with chunk c of row r stored at
+// chunk c ^ (r & 7): the 16 rows read together by a fragment load hit 8
+// different chunk slots, so no padding is needed (2 workgroups fit per CU).
+__device__ __forceinline__ int swz(int r, int chunk) { return r * C + ((chunk ^ (r & 7)) << 3); }
+
+template <bool STORE_Z>
+__global__ __launch_bounds__(NT) void k_mid_fwd_wide(const __bf16* __restrict__ Ain,
+                                                     const __bf16* __restrict__ Wf,
+                                                     const float* __restrict__ bias, float* __restrict__ Z,
+                                                     __bf16* __restrict__ Aout, int X)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 sA[(TILE_F + 4) * C];
+    __shared__ __attribute__((aligned(16))) __bf16 sW[TAPS * C * C];
+    const int b = blockIdx.y, x0 = blockIdx.x * TILE_F;
+    const __bf16* Ab = Ain + (int64_t)b * X * C;
+    for (int q = threadIdx.x; q < (TILE_F + 4) * (C / 8); q += NT) {
+        const int r = q / (C / 8), c8 = q % (C / 8);
+        const int x = wrapx(x0 - 2 + r, X);
+        *reinterpret_cast<uint4*>(&sA[swz(r, c8)]) =
+            *reinterpret_cast<const uint4*>(&Ab[(int64_t)x * C + c8 * 8]);
+    }
+    for (int q = threadIdx.x; q < TAPS * C * (C / 8); q += NT) {
+        const int r = q / (C / 8), c8 = q % (C / 8);
+        *reinterpret_cast<uint4*>(&sW[swz(r, c8)]) = *reinterpret_cast<const uint4*>(&Wf[r * C + c8 * 8]);
+    }
+    __syncthreads();
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int m0 = 64 * w, li = l % 16, lq = l / 16;
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb) acc[mb][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+    for (int s = 0; s < TAPS * 2; ++s) {
+        const int t = s >> 1, chunk = (s & 1) * 4 + lq;  // K = channels (s&1)*32 + 8*lq ..
+        bf16x8 a[4], bv[4];
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb)
+            a[mb] = *reinterpret_cast<const bf16x8*>(&sA[swz(m0 + 16 * mb + li + t, chunk)]);
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb)
+            bv[nb] = *reinterpret_cast<const bf16x8*>(&sW[swz(t * C + 16 * nb + li, chunk)]);
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+            for (int nb = 0; nb < 4; ++nb)
+                acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mb], bv[nb], acc[mb][nb], 0, 0, 0);
+    }
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb) {
+        const int co = 16 * nb + li;
+        const float bc = bias[co];
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int x = x0 + m0 + 16 * mb + 4 * lq + j;
+                const float z = acc[mb][nb][j] + bc;
+                const int64_t o = ((int64_t)b * X + x) * C + co;
+                if (STORE_Z) Z[o] = z;
+                Aout[o] = f2bf(gelu(z));
+            }
+    }
+}
+
 // dA[b][y][ci] = sum_t sum_co dZ[b][y-t+2][co] W[t][co][ci]   (W as Wb[t][ci][co])
 // then dZprev = bf16(dA * gelu'(Zprev)).
 __global__ __launch_bounds__(NT) void k_mid_bwd_data(const __bf16* __restrict__ dZ, const __bf16* __restrict__ Wb,
@@ -508,8 +579,8 @@ int pack_weights(sw1d_nn* N)
     return SW1D_OK;
 }
 
-// forward of B samples already in N->x
-int nn_forward(sw1d_nn* N, int B)
+// forward of B samples already in N->x (store_z: keep pre-activations for backward)
+int nn_forward(sw1d_nn* N, int B, bool store_z = true)
 {
     const int X = N->X;
     const int64_t nfeat = (int64_t)B * X * C;
@@ -517,8 +588,17 @@ int nn_forward(sw1d_nn* N, int B)
                        N->p + P_B1, N->Z[0], N->A[0], X, B);
     for (int l = 0; l < 4; ++l) {
         const float* bias = N->p + P_MID + l * MID_SZ + TAPS * C * C;
-        hipLaunchKernelGGL(k_mid_fwd, dim3(X / TILE, B), dim3(NT), 0, N->stream, N->A[l], N->wf[l], bias,
-                           N->Z[l + 1], N->A[l + 1], X);
+        if (X % TILE_F == 0) {
+            if (store_z)
+                hipLaunchKernelGGL(k_mid_fwd_wide<true>, dim3(X / TILE_F, B), dim3(NT), 0, N->stream, N->A[l],
+                                   N->wf[l], bias, N->Z[l + 1], N->A[l + 1], X);
+            else
+                hipLaunchKernelGGL(k_mid_fwd_wide<false>, dim3(X / TILE_F, B), dim3(NT), 0, N->stream, N->A[l],
+                                   N->wf[l], bias, N->Z[l + 1], N->A[l + 1], X);
+        } else {
+            hipLaunchKernelGGL(k_mid_fwd, dim3(X / TILE, B), dim3(NT), 0, N->stream, N->A[l], N->wf[l], bias,
+                               N->Z[l + 1], N->A[l + 1], X);
+        }
     }
     hipLaunchKernelGGL(k_l6_fwd, dim3(nblk((int64_t)B * X)), dim3(NT), 0, N->stream, N->A[4],
                        N->p + P_W6, N->p + P_B6, N->du, X, B);
@@ -817,7 +897,7 @@ extern "C" int sw1d_surrogate_advance(sw1d_nn* N, sw1d_batch* batch, int64_t sur
         if (judge) NN_HIP(hipMemcpyAsync(N->usave, u, bytes, hipMemcpyDeviceToDevice, N->stream));
         hipLaunchKernelGGL(k_to_input, dim3(blocks), dim3(NT), 0, N->stream, u, N->x, len,
                            (float)(1.0 / N->in_scale));
-        int rc = nn_forward(N, (int)B);
+        int rc = nn_forward(N, (int)B, false);
         if (rc) return rc;
         hipLaunchKernelGGL(k_apply_du, dim3(blocks), dim3(NT), 0, N->stream, u, N->du, len, N->out_scale);
         NN_HIP(hipGetLastError());
@@ -1018,7 +1098,7 @@ int sw1d_nn_eval(sw1d_nn* N, int64_t first, int64_t count, double* rel_err)
     for (int64_t s = first; s < first + count; s += N->Bmax) {
         const int b = (int)std::min<int64_t>(N->Bmax, first + count - s);
         NN_HIP(hipMemcpyAsync(N->x, D->x + s * row, b * row * sizeof(float), hipMemcpyDeviceToDevice, N->stream));
-        int rc = nn_forward(N, b);
+        int rc = nn_forward(N, b, false);
         if (rc) return rc;
         y.resize(b * row);
         t.resize(b * row);

@@ -139,6 +139,7 @@ def main():
     if not args.no_extras:
         out["trapezoid"] = _trapezoid(sw, n, dt, rng_ic, args)
         out["batched"] = _batched(sw)
+        out["surrogate"] = _surrogate()
     if not args.no_cpu:
         out["cpu_baseline"] = _cpu_baseline(n, dt, rng_ic, args.cpu_seconds)
     print(json.dumps(out))
@@ -211,6 +212,29 @@ def _ks_ic(n, seed):
     base = 2.0 * np.cos(19.0 * x * np.pi / 128.0)
     rng = np.random.default_rng(seed)
     return base + 1e-3 * rng.standard_normal(n)
+
+
+def _surrogate():
+    """Learned KS surrogate (nX=1024, K=20): trained on the GPU from 512 solver
+    trajectories of 10000 steps, then a judged rollout (every 10th network
+    step checked against the plain solver, tolerance 1e-2 relative L2) on 256
+    unseen trajectories; throughput in point-update equivalents (problems x
+    nX x K per network step)."""
+    sur = importlib.import_module("1dsweptcuda_amd.surrogate")
+    t0 = time.perf_counter()
+    nn, rep = sur.train(ntraj=512, steps=10000, iters=3000, log=lambda *_: None)
+    ev = sur.evaluate(nn, ntraj=256)
+    nn.close()
+    return {"held_out_rel_err_du": rep["held_out_rel_err_du"], "train_seconds": rep["train_seconds"],
+            "dataset_samples": rep["samples"], "judged_rollout": ev["judged_rollout"],
+            "one_step_rel_l2": ev["one_step_rel_l2"],
+            "value": ev["surrogate_pu_per_s"], "unit": "point-update equivalents/s",
+            "unjudged_value": ev["surrogate_unjudged_pu_per_s"],
+            "plain_single_swept_nx1024": ev["plain_single_swept_pu_per_s"],
+            "plain_batch_same_problems": ev["plain_batch_pu_per_s"],
+            "speedup_vs_single_swept": ev["speedup_vs_single_swept"],
+            "speedup_vs_plain_batch": ev["speedup_vs_plain_batch"],
+            "seconds": time.perf_counter() - t0}
 
 
 def _cpu_baseline(n, dt, u0, budget_s):
