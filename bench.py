@@ -36,6 +36,10 @@ KS_FLOP_PER_PU = 40          # stutterStep 20 + finalStep 20 (KS1D_SweptShared.c
 KS_SWEPT_BYTES_PER_PU = 256  # per tile launch: (u, u_half) read + written, 32 B / (W/8 steps) * W
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per whole-diamond launch (nX, node width) -> bytes, from rocprofv3
+# FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by its reading for a copy
+# kernel of known size (profiles/ks_swept_diamond_nx1048576_hbm_bytes.md).
+DIAMOND_TRAFFIC_BYTES = {(1 << 20, 512): 33.96e6}
 
 
 def _timed(plan, steps):
@@ -58,12 +62,18 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the trapezoid engine and the batched workload")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="first run one plain copy kernel over this many MiB (reads and writes "
+                         "that many bytes), the known-size reference for HBM byte counters")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(the solver runs on a single device, no collectives)", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration_mib:
+        _copy_calibration(args.copy_calibration_mib)
 
     sw = importlib.import_module("1dsweptcuda_amd")
     n, dt = args.nx, 0.005
@@ -131,7 +141,8 @@ def main():
         "roofline": {"bound": "mfma", "compute": "fp64 vector ALU (no MFMA); peak = MI355X FP64",
                      "kernel": "k_ks_swept<double,%d,DIAMOND>" % (W // 64),
                      "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved_tf / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved_tf / FP64_PEAK_TFLOPS,
+                     "traffic": DIAMOND_TRAFFIC_BYTES.get((n, W)),
                      "avg_launch_us": avg_ms * 1e3, "launches": launches,
                      "flop_per_launch": flops,
                      "algorithmic_bytes_per_launch": 32 * n},
@@ -143,6 +154,18 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = _cpu_baseline(n, dt, rng_ic, args.cpu_seconds)
     print(json.dumps(out))
+
+
+def _copy_calibration(mib):
+    """One elementwise kernel that reads `mib` MiB and writes `mib` MiB
+    (torch.neg over float32), for scaling FETCH_SIZE / WRITE_SIZE readings."""
+    import torch
+    a = torch.ones(mib << 18, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    torch.neg(a, out=b)
+    torch.cuda.synchronize()
+    del a, b
 
 
 def _trapezoid(sw, n, dt, u0, args):
