@@ -54,15 +54,38 @@ __host__ __device__ inline float bf_bits2f(uint16_t h)
     memcpy(&f, &u, 4);
     return f;
 }
-__device__ inline __bf16 f2bf(float f) { return __builtin_bit_cast(__bf16, f2bf_bits(f)); }
+// Device rounding uses the hardware conversion (v_cvt_pk_bf16_f32, round to
+// nearest even); the device check compares it with f2bf_bits on ties and
+// neighbours of ties.
+__device__ inline __bf16 f2bf(float f) { return static_cast<__bf16>(f); }
 __device__ inline float bf2f(__bf16 h) { return bf_bits2f(__builtin_bit_cast(uint16_t, h)); }
 
-// GELU(z) = z/2 (1 + erf(z/sqrt 2)) and its derivative.
-__device__ inline float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+// erf(x) by Abramowitz & Stegun 7.1.26 (Hastings' form): for a = |x|,
+//   erf(a) = 1 - t (a1 + t (a2 + t (a3 + t (a4 + t a5)))) exp(-a^2),  t = 1 / (1 + p a),
+// |error| <= 1.5e-7 for all a (plus FP32 rounding of the evaluation), with no
+// branches: two transcendental instructions (reciprocal, 2^x) and seven FMAs.
+__device__ inline float erf_as(float x)
+{
+    const float a = fabsf(x);
+    const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, a, 1.0f));
+    float q = fmaf(1.061405429f, t, -1.453152027f);
+    q = fmaf(q, t, 1.421413741f);
+    q = fmaf(q, t, -0.284496736f);
+    q = fmaf(q, t, 0.254829592f);
+    const float e = __builtin_amdgcn_exp2f(-1.4426950408889634f * (a * a));  // exp(-a^2)
+    return copysignf(fmaf(-q * t, e, 1.0f), x);
+}
+
+// GELU(z) = z/2 (1 + erf(z/sqrt 2)) and its derivative Phi(z) + z phi(z).
+__device__ inline float gelu(float z)
+{
+    const float h = 0.5f * z;
+    return fmaf(h, erf_as(z * 0.70710678118654752f), h);
+}
 __device__ inline float gelu_d(float z)
 {
-    return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) +
-           z * 0.39894228040143268f * expf(-0.5f * z * z);
+    const float phi = 0.39894228040143268f * __builtin_amdgcn_exp2f(-0.72134752044448170f * (z * z));
+    return fmaf(z, phi, 0.5f * (1.0f + erf_as(z * 0.70710678118654752f)));
 }
 
 __device__ inline int wrapx(int x, int X) { return x < 0 ? x + X : (x >= X ? x - X : x); }
@@ -91,24 +114,34 @@ __global__ void k_mfma_check(int* ok)
             want += (float)((m * 3 + k * 5) % 7 - 3) * (float)((k * 2 + n * 7) % 5 - 2);
         if (d[j] != want) atomicAnd(ok, 0);
     }
+    // hardware BF16 rounding against the software rule: ties (low half 0x8000)
+    // with even and odd kept bits, and their neighbours, over many exponents
+    const uint32_t lows[6] = {0x0000u, 0x0001u, 0x7fffu, 0x8000u, 0x8001u, 0xffffu};
+    for (int k = 0; k < 6; ++k) {
+        const uint32_t bits = ((uint32_t)(l & 1) << 31) | ((uint32_t)(64 + 2 * l) << 23) |
+                              ((uint32_t)(l * 37 % 128) << 16) | lows[k];
+        float f;
+        memcpy(&f, &bits, 4);
+        if (__builtin_bit_cast(uint16_t, f2bf(f)) != f2bf_bits(f)) atomicAnd(ok, 0);
+    }
 }
 
 // ------------------------------------------------------ L1: 1 -> 64, VALU ---
 // Z1[b][x][c] = b1[c] + sum_t w1[c][t] u[b][x+t-2];  A1 = bf16(gelu(Z1))
 __global__ __launch_bounds__(NT) void k_l1_fwd(const float* __restrict__ u, const float* __restrict__ w1,
                                                const float* __restrict__ b1, float* __restrict__ Z,
-                                               __bf16* __restrict__ A, int X, int B)
+                                               __bf16* __restrict__ A, int X)
 {
-    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;  // (b, x, c)
-    if (i >= (int64_t)B * X * C) return;
-    const int c = (int)(i % C);
-    const int64_t bx = i / C;
-    const int x = (int)(bx % X), b = (int)(bx / X);
+    const int b = blockIdx.y;
+    const int i = blockIdx.x * NT + threadIdx.x;  // x * C + c within sample b
+    if (i >= X * C) return;
+    const int c = i % C, x = i / C;
     const float* ub = u + (int64_t)b * X;
     float z = b1[c];
-    for (int t = 0; t < TAPS; ++t) z += w1[c * TAPS + t] * ub[wrapx(x + t - 2, X)];
-    Z[i] = z;
-    A[i] = f2bf(gelu(z));
+    for (int t = 0; t < TAPS; ++t) z = fmaf(w1[c * TAPS + t], ub[wrapx(x + t - 2, X)], z);
+    const int64_t o = (int64_t)b * X * C + i;
+    Z[o] = z;
+    A[o] = f2bf(gelu(z));
 }
 
 // ----------------------------------------------- L2-L5: 64 -> 64, MFMA ---
@@ -160,37 +193,66 @@ __global__ __launch_bounds__(NT) void k_mid_fwd(const __bf16* __restrict__ Ain, 
 // Same product over 256 positions per workgroup: the 40 KB of staged weights
 // serve four times the positions, each wave owns 64 positions x 64 channels
 // (16 accumulator tiles) and reuses every A and B fragment four times.
-// STORE_Z keeps the pre-activations for the backward pass (training only).
 constexpr int TILE_F = 256;
+static_assert(NT == TILE_F, "the output-layer epilogue maps one thread to one position");
 // LDS rows of 64 BF16 (8 chunks of 16 bytes) with chunk c of row r stored at
 // chunk c ^ (r & 7): the 16 rows read together by a fragment load hit 8
 // different chunk slots, so no padding is needed (2 workgroups fit per CU).
 __device__ __forceinline__ int swz(int r, int chunk) { return r * C + ((chunk ^ (r & 7)) << 3); }
 
-template <bool STORE_Z>
-__global__ __launch_bounds__(NT) void k_mid_fwd_wide(const __bf16* __restrict__ Ain,
-                                                     const __bf16* __restrict__ Wf,
-                                                     const float* __restrict__ bias, float* __restrict__ Z,
-                                                     __bf16* __restrict__ Aout, int X)
+// rows x0-2 .. x0+TILE_F+1 of one sample's [X][64] BF16 activation -> sA
+__device__ __forceinline__ void stage_act(__bf16* sA, const __bf16* __restrict__ Ab, int x0, int X)
 {
-    __shared__ __attribute__((aligned(16))) __bf16 sA[(TILE_F + 4) * C];
-    __shared__ __attribute__((aligned(16))) __bf16 sW[TAPS * C * C];
-    const int b = blockIdx.y, x0 = blockIdx.x * TILE_F;
-    const __bf16* Ab = Ain + (int64_t)b * X * C;
     for (int q = threadIdx.x; q < (TILE_F + 4) * (C / 8); q += NT) {
         const int r = q / (C / 8), c8 = q % (C / 8);
         const int x = wrapx(x0 - 2 + r, X);
         *reinterpret_cast<uint4*>(&sA[swz(r, c8)]) =
             *reinterpret_cast<const uint4*>(&Ab[(int64_t)x * C + c8 * 8]);
     }
+}
+// a layer's packed weights [t][co][ci] -> sW (rows t*64 + co)
+__device__ __forceinline__ void stage_w(__bf16* sW, const __bf16* __restrict__ Wf)
+{
     for (int q = threadIdx.x; q < TAPS * C * (C / 8); q += NT) {
         const int r = q / (C / 8), c8 = q % (C / 8);
         *reinterpret_cast<uint4*>(&sW[swz(r, c8)]) = *reinterpret_cast<const uint4*>(&Wf[r * C + c8 * 8]);
     }
+}
+// Layer 1 evaluated straight into sA: rows x0-2 .. x0+TILE_F+1 of
+// bf16(gelu(b1 + sum_t w1[c][t] in[x+t-2])), with in = FP32(u) * inv_scale
+// (the same operations and order as k_l1_fwd and k_to_input, so the same bits).
+template <typename TIN>
+__device__ __forceinline__ void stage_l1(__bf16* sA, float* sIn, const TIN* __restrict__ ub, float inv_scale,
+                                         const float* __restrict__ w1, const float* __restrict__ b1, int x0,
+                                         int X)
+{
+    for (int q = threadIdx.x; q < TILE_F + 8; q += NT) sIn[q] = (float)ub[wrapx(x0 - 4 + q, X)] * inv_scale;
+    const int c8 = threadIdx.x % (C / 8);  // fixed per thread: NT is a multiple of 8
+    float w[8][TAPS], bb[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        bb[k] = b1[c8 * 8 + k];
+#pragma unroll
+        for (int t = 0; t < TAPS; ++t) w[k][t] = w1[(c8 * 8 + k) * TAPS + t];
+    }
     __syncthreads();
+    for (int r = threadIdx.x / (C / 8); r < TILE_F + 4; r += NT / (C / 8)) {
+        bf16x8 v;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            float z = bb[k];
+#pragma unroll
+            for (int t = 0; t < TAPS; ++t) z = fmaf(w[k][t], sIn[r + t], z);
+            v[k] = f2bf(gelu(z));
+        }
+        *reinterpret_cast<bf16x8*>(&sA[swz(r, c8)]) = v;
+    }
+}
+// acc[mb][nb] = the wave's 64 positions x 64 channels of sum_t A(x+t-2) W[t]^T
+__device__ __forceinline__ void mfma_tile(const __bf16* sA, const __bf16* sW, f32x4 (&acc)[4][4])
+{
     const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int m0 = 64 * w, li = l % 16, lq = l / 16;
-    f32x4 acc[4][4];
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
@@ -211,6 +273,16 @@ __global__ __launch_bounds__(NT) void k_mid_fwd_wide(const __bf16* __restrict__ 
             for (int nb = 0; nb < 4; ++nb)
                 acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mb], bv[nb], acc[mb][nb], 0, 0, 0);
     }
+}
+// bias + GELU of the accumulators, rounded to BF16 into the LDS tile sO (row =
+// position in the tile, swizzled); FP32 pre-activations to Z (rows zrow0 + r)
+// when STORE_Z.  sO may alias sA once every wave has finished mfma_tile.
+template <bool STORE_Z>
+__device__ __forceinline__ void epilogue(const f32x4 (&acc)[4][4], const float* __restrict__ bias, __bf16* sO,
+                                         float* __restrict__ Z, int64_t zrow0)
+{
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int m0 = 64 * w, li = l % 16, lq = l / 16;
 #pragma unroll
     for (int nb = 0; nb < 4; ++nb) {
         const int co = 16 * nb + li;
@@ -219,13 +291,124 @@ __global__ __launch_bounds__(NT) void k_mid_fwd_wide(const __bf16* __restrict__ 
         for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const int x = x0 + m0 + 16 * mb + 4 * lq + j;
+                const int r = m0 + 16 * mb + 4 * lq + j;
                 const float z = acc[mb][nb][j] + bc;
-                const int64_t o = ((int64_t)b * X + x) * C + co;
-                if (STORE_Z) Z[o] = z;
-                Aout[o] = f2bf(gelu(z));
+                if (STORE_Z) Z[(zrow0 + r) * C + co] = z;
+                sO[swz(r, co >> 3) + (co & 7)] = f2bf(gelu(z));
             }
     }
+}
+// the LDS tile -> rows x0 .. x0+TILE_F-1 of a [X][64] activation (16-byte stores)
+__device__ __forceinline__ void store_act(const __bf16* sO, __bf16* __restrict__ Ab, int x0)
+{
+    for (int q = threadIdx.x; q < TILE_F * (C / 8); q += NT) {
+        const int r = q / (C / 8), c8 = q % (C / 8);
+        *reinterpret_cast<uint4*>(&Ab[(int64_t)(x0 + r) * C + c8 * 8]) =
+            *reinterpret_cast<const uint4*>(&sO[swz(r, c8)]);
+    }
+}
+
+// One 64 -> 64 layer over 256 positions.  STORE_Z keeps the pre-activations
+// for the backward pass (training only).
+template <bool STORE_Z>
+__global__ __launch_bounds__(NT) void k_mid_fwd_wide(const __bf16* __restrict__ Ain,
+                                                     const __bf16* __restrict__ Wf,
+                                                     const float* __restrict__ bias, float* __restrict__ Z,
+                                                     __bf16* __restrict__ Aout, int X)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 sA[(TILE_F + 4) * C];
+    __shared__ __attribute__((aligned(16))) __bf16 sW[TAPS * C * C];
+    const int b = blockIdx.y, x0 = blockIdx.x * TILE_F;
+    stage_act(sA, Ain + (int64_t)b * X * C, x0, X);
+    stage_w(sW, Wf);
+    __syncthreads();
+    f32x4 acc[4][4];
+    mfma_tile(sA, sW, acc);
+    __syncthreads();
+    epilogue<STORE_Z>(acc, bias, sA, Z, (int64_t)b * X + x0);
+    __syncthreads();
+    store_act(sA, Aout + (int64_t)b * X * C, x0);
+}
+
+// Layers 1 and 2 in one pass (inference): layer 1 is evaluated into the LDS
+// tile from the network input, so its activation never goes to memory.
+template <typename TIN>
+__global__ __launch_bounds__(NT) void k_l12_fwd(const TIN* __restrict__ in, float inv_scale,
+                                                const float* __restrict__ w1, const float* __restrict__ b1,
+                                                const __bf16* __restrict__ Wf, const float* __restrict__ bias,
+                                                __bf16* __restrict__ Aout, int X)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 sA[(TILE_F + 4) * C];
+    __shared__ __attribute__((aligned(16))) __bf16 sW[TAPS * C * C];
+    __shared__ float sIn[TILE_F + 8];
+    const int b = blockIdx.y, x0 = blockIdx.x * TILE_F;
+    stage_w(sW, Wf);
+    stage_l1<TIN>(sA, sIn, in + (int64_t)b * X, inv_scale, w1, b1, x0, X);
+    __syncthreads();
+    f32x4 acc[4][4];
+    mfma_tile(sA, sW, acc);
+    __syncthreads();
+    epilogue<false>(acc, bias, sA, nullptr, 0);
+    __syncthreads();
+    store_act(sA, Aout + (int64_t)b * X * C, x0);
+}
+
+// Layer 5 and the output layer (inference): the BF16 layer-5 tile stays in
+// LDS and each thread forms, for its position x, the five tap sums
+//   S[t][b*X + x] = sum_c w6[t][c] A5[x][c],
+// from which du[x] = b6 + sum_t S[t][x+t-2] (k_l6_out).
+__global__ __launch_bounds__(NT) void k_l56_fwd(const __bf16* __restrict__ Ain, const __bf16* __restrict__ Wf,
+                                                const float* __restrict__ bias, const float* __restrict__ w6,
+                                                float* __restrict__ S, int X, int64_t plane)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 sA[(TILE_F + 4) * C];
+    __shared__ __attribute__((aligned(16))) __bf16 sW[TAPS * C * C];
+    __shared__ float sW6[TAPS * C];
+    const int b = blockIdx.y, x0 = blockIdx.x * TILE_F;
+    stage_act(sA, Ain + (int64_t)b * X * C, x0, X);
+    stage_w(sW, Wf);
+    for (int q = threadIdx.x; q < TAPS * C; q += NT) sW6[q] = w6[q];
+    __syncthreads();
+    f32x4 acc[4][4];
+    mfma_tile(sA, sW, acc);
+    __syncthreads();
+    epilogue<false>(acc, bias, sA, nullptr, 0);
+    __syncthreads();
+    const int r = threadIdx.x;
+    float sum[TAPS] = {0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int c8 = 0; c8 < C / 8; ++c8) {
+        const bf16x8 v = *reinterpret_cast<const bf16x8*>(&sA[swz(r, c8)]);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const float a = bf2f(v[k]);
+#pragma unroll
+            for (int t = 0; t < TAPS; ++t) sum[t] = fmaf(sW6[t * C + c8 * 8 + k], a, sum[t]);
+        }
+    }
+    const int64_t o = (int64_t)b * X + x0 + r;
+#pragma unroll
+    for (int t = 0; t < TAPS; ++t) S[t * plane + o] = sum[t];
+}
+
+// du = b6 + sum_t S[t][x+t-2] (periodic per sample); APPLY adds out_scale * du
+// to the FP64 state in place of storing du (the operation of k_apply_du).
+template <bool APPLY>
+__global__ __launch_bounds__(NT) void k_l6_out(const float* __restrict__ S, const float* __restrict__ b6,
+                                               float* __restrict__ du, double* __restrict__ u,
+                                               double out_scale, int X, int64_t plane)
+{
+    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (i >= plane) return;
+    const int x = (int)(i % X);
+    const int64_t row = i - x;
+    float s = b6[0];
+#pragma unroll
+    for (int t = 0; t < TAPS; ++t) s += S[t * plane + row + wrapx(x + t - 2, X)];
+    if (APPLY)
+        u[i] += out_scale * (double)s;
+    else
+        du[i] = s;
 }
 
 // dA[b][y][ci] = sum_t sum_co dZ[b][y-t+2][co] W[t][co][ci]   (W as Wb[t][ci][co])
@@ -322,31 +505,38 @@ __global__ __launch_bounds__(NT) void k_mid_bwd_weight(const __bf16* __restrict_
 }
 
 // out[i] = sum_{w < nparts} part[w][i], fixed order.
+// Sum of nparts partial vectors of length len in two fixed-order passes:
+// group g = blockIdx.y adds partials [g*per, (g+1)*per) into out[g] (first
+// pass, many groups in flight), then one group adds the group sums.
 __global__ __launch_bounds__(NT) void k_reduce_parts(const float* __restrict__ part, int64_t nparts,
-                                                     int64_t len, float* __restrict__ out)
+                                                     int64_t len, int64_t per, float* __restrict__ out)
 {
     const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
     if (i >= len) return;
+    const int64_t w0 = (int64_t)blockIdx.y * per;
+    const int64_t w1 = w0 + per < nparts ? w0 + per : nparts;
     float s = 0.f;
-    for (int64_t w = 0; w < nparts; ++w) s += part[w * len + i];
-    out[i] = s;
+    for (int64_t w = w0; w < w1; ++w) s += part[w * len + i];
+    out[(int64_t)blockIdx.y * len + i] = s;
 }
 
 // ------------------------------------------------------ L6: 64 -> 1, VALU ---
 // du[b][x] = b6 + sum_t sum_ci w6[t][ci] A5[b][x+t-2][ci]
 __global__ __launch_bounds__(NT) void k_l6_fwd(const __bf16* __restrict__ A, const float* __restrict__ w6,
-                                               const float* __restrict__ b6, float* __restrict__ du, int X, int B)
+                                               const float* __restrict__ b6, float* __restrict__ du, int X)
 {
-    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
-    if (i >= (int64_t)B * X) return;
-    const int x = (int)(i % X), b = (int)(i / X);
+    const int b = blockIdx.y, x = blockIdx.x * NT + threadIdx.x;
+    if (x >= X) return;
     const __bf16* Ab = A + (int64_t)b * X * C;
     float s = b6[0];
     for (int t = 0; t < TAPS; ++t) {
-        const __bf16* row = Ab + (int64_t)wrapx(x + t - 2, X) * C;
-        for (int c = 0; c < C; ++c) s += w6[t * C + c] * bf2f(row[c]);
+        const bf16x8* row = reinterpret_cast<const bf16x8*>(Ab + (int64_t)wrapx(x + t - 2, X) * C);
+        for (int c8 = 0; c8 < C / 8; ++c8) {
+            const bf16x8 v = row[c8];
+            for (int k = 0; k < 8; ++k) s = fmaf(w6[t * C + c8 * 8 + k], bf2f(v[k]), s);
+        }
     }
-    du[i] = s;
+    du[(int64_t)b * X + x] = s;
 }
 
 // loss = mean (du - target)^2 over the batch (partials per block) and
@@ -468,6 +658,26 @@ __global__ __launch_bounds__(NT) void k_apply_du(double* __restrict__ u, const f
     if (i < n) u[i] += scale * (double)du[i];
 }
 
+// Judged step: u[p] <- unn[p] for the problems whose network state is kept.
+__global__ __launch_bounds__(NT) void k_keep_rows(double* __restrict__ u, const double* __restrict__ unn,
+                                                  const int* __restrict__ keep, int X, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (i < n && keep[i / X]) u[i] = unn[i];
+}
+
+// Training mini-batch: row b of (x, t) <- sample idx[b] of the stored set.
+__global__ __launch_bounds__(NT) void k_gather(const float* __restrict__ sx, const float* __restrict__ st,
+                                               const int64_t* __restrict__ idx, float* __restrict__ x,
+                                               float* __restrict__ t, int X, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (i >= n) return;
+    const int64_t b = i / X, src = idx[b] * X + (i - b * X);
+    x[i] = sx[src];
+    t[i] = st[src];
+}
+
 // Relative L2 distance ||a - b|| / ||b|| partial sums per block: (sum (a-b)^2, sum b^2).
 __global__ __launch_bounds__(NT) void k_l2_parts(const double* __restrict__ a, const double* __restrict__ b,
                                                  int64_t n, double* __restrict__ part)
@@ -508,6 +718,7 @@ constexpr int64_t P_W6 = P_MID + 4 * MID_SZ, P_B6 = P_W6 + TAPS * C;
 constexpr int64_t NPARAM = P_B6 + 1;
 
 inline unsigned nblk(int64_t n) { return (unsigned)((n + NT - 1) / NT); }
+constexpr int64_t RED_GROUPS = 32;  // first-pass groups of the gradient reduction
 
 uint64_t splitmix(uint64_t& s)
 {
@@ -558,8 +769,11 @@ struct sw1d_nn {
     __bf16* A[5] = {};      // [B][X][C] activations of L1..L5
     __bf16 *dZa = nullptr, *dZb = nullptr;  // [B][X][C] gradients
     float* part = nullptr;  // per-tile partial gradients
+    float* part2 = nullptr; // group sums of the partials
     float* lossp = nullptr;
     int64_t part_len = 0;
+    float* taps = nullptr;  // [5][B][X] output-layer tap sums (inference)
+    int* keep = nullptr;    // [B] judged-step selection
     // surrogate integration scratch
     double* usave = nullptr;
     double* unn = nullptr;
@@ -583,9 +797,8 @@ int pack_weights(sw1d_nn* N)
 int nn_forward(sw1d_nn* N, int B, bool store_z = true)
 {
     const int X = N->X;
-    const int64_t nfeat = (int64_t)B * X * C;
-    hipLaunchKernelGGL(k_l1_fwd, dim3(nblk(nfeat)), dim3(NT), 0, N->stream, N->x, N->p + P_W1,
-                       N->p + P_B1, N->Z[0], N->A[0], X, B);
+    hipLaunchKernelGGL(k_l1_fwd, dim3(nblk((int64_t)X * C), B), dim3(NT), 0, N->stream, N->x, N->p + P_W1,
+                       N->p + P_B1, N->Z[0], N->A[0], X);
     for (int l = 0; l < 4; ++l) {
         const float* bias = N->p + P_MID + l * MID_SZ + TAPS * C * C;
         if (X % TILE_F == 0) {
@@ -600,10 +813,65 @@ int nn_forward(sw1d_nn* N, int B, bool store_z = true)
                                N->Z[l + 1], N->A[l + 1], X);
         }
     }
-    hipLaunchKernelGGL(k_l6_fwd, dim3(nblk((int64_t)B * X)), dim3(NT), 0, N->stream, N->A[4],
-                       N->p + P_W6, N->p + P_B6, N->du, X, B);
+    hipLaunchKernelGGL(k_l6_fwd, dim3(nblk(X), B), dim3(NT), 0, N->stream, N->A[4], N->p + P_W6,
+                       N->p + P_B6, N->du, X);
     NN_HIP(hipGetLastError());
     return SW1D_OK;
+}
+
+// Inference of B samples.  u == nullptr: from the FP32 inputs in N->x to du
+// in N->du.  Otherwise from the FP64 states u (normalised on the fly by
+// 1/in_scale) to u += out_scale * du.  Five launches when nx is a multiple of
+// the 256-position tile; otherwise the training kernels plus the conversions.
+int nn_infer(sw1d_nn* N, int B, double* u)
+{
+    const int X = N->X;
+    const int64_t plane = (int64_t)B * X;
+    if (X % TILE_F != 0) {
+        if (u)
+            hipLaunchKernelGGL(k_to_input, dim3(nblk(plane)), dim3(NT), 0, N->stream, u, N->x, plane,
+                               (float)(1.0 / N->in_scale));
+        int rc = nn_forward(N, B, false);
+        if (rc) return rc;
+        if (u)
+            hipLaunchKernelGGL(k_apply_du, dim3(nblk(plane)), dim3(NT), 0, N->stream, u, N->du, plane,
+                               N->out_scale);
+        NN_HIP(hipGetLastError());
+        return SW1D_OK;
+    }
+    const dim3 grid(X / TILE_F, B);
+    const float* P = N->p;
+    auto bias = [&](int l) { return P + P_MID + l * MID_SZ + TAPS * C * C; };
+    if (u)
+        hipLaunchKernelGGL(k_l12_fwd<double>, grid, dim3(NT), 0, N->stream, u, (float)(1.0 / N->in_scale),
+                           P + P_W1, P + P_B1, N->wf[0], bias(0), N->A[1], X);
+    else
+        hipLaunchKernelGGL(k_l12_fwd<float>, grid, dim3(NT), 0, N->stream, N->x, 1.0f, P + P_W1, P + P_B1,
+                           N->wf[0], bias(0), N->A[1], X);
+    for (int l = 1; l <= 2; ++l)
+        hipLaunchKernelGGL(k_mid_fwd_wide<false>, grid, dim3(NT), 0, N->stream, N->A[l], N->wf[l], bias(l),
+                           nullptr, N->A[l + 1], X);
+    hipLaunchKernelGGL(k_l56_fwd, grid, dim3(NT), 0, N->stream, N->A[3], N->wf[3], bias(3), P + P_W6, N->taps,
+                       X, plane);
+    if (u)
+        hipLaunchKernelGGL(k_l6_out<true>, dim3(nblk(plane)), dim3(NT), 0, N->stream, N->taps, P + P_B6,
+                           nullptr, u, N->out_scale, X, plane);
+    else
+        hipLaunchKernelGGL(k_l6_out<false>, dim3(nblk(plane)), dim3(NT), 0, N->stream, N->taps, P + P_B6,
+                           N->du, nullptr, 0.0, X, plane);
+    NN_HIP(hipGetLastError());
+    return SW1D_OK;
+}
+
+// out[0..len) = sum over the nparts partial vectors in N->part (deterministic)
+void reduce_parts(sw1d_nn* N, int64_t nparts, int64_t len, float* out)
+{
+    const int64_t groups = nparts < RED_GROUPS ? nparts : RED_GROUPS;
+    const int64_t per = (nparts + groups - 1) / groups;
+    hipLaunchKernelGGL(k_reduce_parts, dim3(nblk(len), (unsigned)groups), dim3(NT), 0, N->stream, N->part,
+                       nparts, len, per, N->part2);
+    hipLaunchKernelGGL(k_reduce_parts, dim3(nblk(len), 1), dim3(NT), 0, N->stream, N->part2, groups, len,
+                       groups, out);
 }
 
 // loss (mean square against N->tgt) and the full gradient into N->g
@@ -617,8 +885,7 @@ int nn_backward(sw1d_nn* N, int B, double* loss)
     const int64_t ntiles = (int64_t)(X / TILE) * B;
     // L6
     hipLaunchKernelGGL(k_l6_bwd_weight, tiles, dim3(NT), 0, N->stream, N->gdu, N->A[4], N->part, X);
-    hipLaunchKernelGGL(k_reduce_parts, dim3(nblk(TAPS * C + 1)), dim3(NT), 0, N->stream, N->part, ntiles,
-                       (int64_t)(TAPS * C + 1), N->g + P_W6);
+    reduce_parts(N, ntiles, (int64_t)(TAPS * C + 1), N->g + P_W6);
     hipLaunchKernelGGL(k_l6_bwd_data, dim3(nblk(nfeat)), dim3(NT), 0, N->stream, N->gdu, N->p + P_W6,
                        N->Z[4], N->dZa, X, B);
     // L5 .. L2
@@ -626,15 +893,13 @@ int nn_backward(sw1d_nn* N, int B, double* loss)
     __bf16* dzp = N->dZb;
     for (int l = 3; l >= 0; --l) {
         hipLaunchKernelGGL(k_mid_bwd_weight, tiles, dim3(NT), 0, N->stream, dz, N->A[l], N->part, X);
-        hipLaunchKernelGGL(k_reduce_parts, dim3(nblk(MID_SZ)), dim3(NT), 0, N->stream, N->part, ntiles,
-                           MID_SZ, N->g + P_MID + l * MID_SZ);
+        reduce_parts(N, ntiles, MID_SZ, N->g + P_MID + l * MID_SZ);
         hipLaunchKernelGGL(k_mid_bwd_data, tiles, dim3(NT), 0, N->stream, dz, N->wb[l], N->Z[l], dzp, X);
         __bf16* t = dz; dz = dzp; dzp = t;
     }
     // L1 (dz now holds dZ1)
     hipLaunchKernelGGL(k_l1_bwd_weight, tiles, dim3(NT), 0, N->stream, dz, N->x, N->part, X);
-    hipLaunchKernelGGL(k_reduce_parts, dim3(nblk(C * TAPS + C)), dim3(NT), 0, N->stream, N->part, ntiles,
-                       (int64_t)(C * TAPS + C), N->g + P_W1);
+    reduce_parts(N, ntiles, (int64_t)(C * TAPS + C), N->g + P_W1);
     NN_HIP(hipGetLastError());
     if (loss) {
         std::vector<float> lp(lb);
@@ -661,8 +926,8 @@ void nn_release(sw1d_nn* N)
 {
     if (!N) return;
     if (N->stream) (void)hipStreamSynchronize(N->stream);
-    void* ptrs[] = {N->p, N->g, N->m, N->v, N->x, N->du, N->tgt, N->gdu, N->dZa, N->dZb, N->part,
-                    N->lossp, N->usave, N->unn, N->l2p};
+    void* ptrs[] = {N->p,   N->g,     N->m,     N->v,    N->x,     N->du,   N->tgt,  N->gdu,  N->dZa,
+                    N->dZb, N->part,  N->part2, N->lossp, N->taps, N->keep, N->usave, N->unn, N->l2p};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     for (int l = 0; l < 5; ++l) {
@@ -730,7 +995,9 @@ int sw1d_nn_create(sw1d_nn** out, int32_t nx, int32_t max_batch, uint64_t seed)
               dalloc(&N->tgt, nout) == hipSuccess && dalloc(&N->gdu, nout) == hipSuccess &&
               dalloc(&N->dZa, nfeat) == hipSuccess && dalloc(&N->dZb, nfeat) == hipSuccess &&
               dalloc(&N->part, N->part_len) == hipSuccess &&
-              dalloc(&N->lossp, (int64_t)nblk(nout)) == hipSuccess;
+              dalloc(&N->part2, RED_GROUPS * MID_SZ) == hipSuccess &&
+              dalloc(&N->lossp, (int64_t)nblk(nout)) == hipSuccess &&
+              dalloc(&N->taps, TAPS * nout) == hipSuccess && dalloc(&N->keep, (int64_t)max_batch) == hipSuccess;
     for (int l = 0; ok && l < 5; ++l)
         ok = dalloc(&N->Z[l], nfeat) == hipSuccess && dalloc(&N->A[l], nfeat) == hipSuccess;
     for (int l = 0; ok && l < 4; ++l)
@@ -804,7 +1071,7 @@ int sw1d_nn_forward(sw1d_nn* N, const float* x_host, int32_t batch, float* y_hos
     if (batch < 1 || batch > N->Bmax) return nn_fail(SW1D_EINVAL, "batch out of range");
     const size_t bytes = (size_t)batch * N->X * sizeof(float);
     NN_HIP(hipMemcpyAsync(N->x, x_host, bytes, hipMemcpyHostToDevice, N->stream));
-    int rc = nn_forward(N, batch);
+    int rc = nn_infer(N, batch, nullptr);
     if (rc) return rc;
     NN_HIP(hipMemcpyAsync(y_host, N->du, bytes, hipMemcpyDeviceToHost, N->stream));
     NN_HIP(hipStreamSynchronize(N->stream));
@@ -839,23 +1106,30 @@ int sw1d_nn_train(sw1d_nn* N, const float* d_inputs, const float* d_targets, int
     if (!N || !d_inputs || !d_targets) return nn_fail(SW1D_EINVAL, "null argument");
     if (batch < 1 || batch > N->Bmax || nsamples < 1 || iters < 0)
         return nn_fail(SW1D_EINVAL, "bad batch / sample count");
-    const size_t row = (size_t)N->X * sizeof(float);
+    if (iters == 0) {
+        if (last_loss) *last_loss = 0.0;
+        return SW1D_OK;
+    }
+    // every mini-batch's sample indices, drawn up front and gathered on the device
+    std::vector<int64_t> idx((size_t)(iters * batch));
     uint64_t s = seed;
+    for (auto& k : idx) k = (int64_t)(splitmix(s) % (uint64_t)nsamples);
+    int64_t* didx = nullptr;
+    NN_HIP(hipMalloc((void**)&didx, idx.size() * sizeof(int64_t)));
+    int rc = hipMemcpyAsync(didx, idx.data(), idx.size() * sizeof(int64_t), hipMemcpyHostToDevice,
+                            N->stream) == hipSuccess ? SW1D_OK : nn_fail(SW1D_EDEVICE, "index upload");
     double loss = 0;
-    for (int64_t it = 0; it < iters; ++it) {
-        for (int b = 0; b < batch; ++b) {
-            const int64_t k = (int64_t)(splitmix(s) % (uint64_t)nsamples);
-            NN_HIP(hipMemcpyAsync((char*)N->x + b * row, (const char*)d_inputs + k * row, row,
-                                  hipMemcpyDeviceToDevice, N->stream));
-            NN_HIP(hipMemcpyAsync((char*)N->tgt + b * row, (const char*)d_targets + k * row, row,
-                                  hipMemcpyDeviceToDevice, N->stream));
-        }
-        int rc = nn_forward(N, batch);
+    const int64_t n = (int64_t)batch * N->X;
+    for (int64_t it = 0; it < iters && !rc; ++it) {
+        hipLaunchKernelGGL(k_gather, dim3(nblk(n)), dim3(NT), 0, N->stream, d_inputs, d_targets,
+                           didx + it * batch, N->x, N->tgt, N->X, n);
+        rc = nn_forward(N, batch);
         if (!rc) rc = nn_backward(N, batch, (it == iters - 1 || it % 50 == 0) ? &loss : nullptr);
         if (!rc) rc = adam(N, lr, 0.9f, 0.999f, 1e-8f);
-        if (rc) return rc;
     }
-    NN_HIP(hipStreamSynchronize(N->stream));
+    if (hipStreamSynchronize(N->stream) != hipSuccess && !rc) rc = nn_fail(SW1D_EDEVICE, "training");
+    (void)hipFree(didx);
+    if (rc) return rc;
     if (last_loss) *last_loss = loss;
     return SW1D_OK;
 }
@@ -891,16 +1165,13 @@ extern "C" int sw1d_surrogate_advance(sw1d_nn* N, sw1d_batch* batch, int64_t sur
     int64_t judged = 0, fallbacks = 0;
     double max_err = 0.0;
     std::vector<double> parts(2 * (size_t)blocks);
+    std::vector<int> keep((size_t)B);
     NN_HIP(hipStreamSynchronize(bv.stream));
     for (int64_t i = 1; i <= surrogate_steps; ++i) {
         const bool judge = judge_every > 0 && i % judge_every == 0;
         if (judge) NN_HIP(hipMemcpyAsync(N->usave, u, bytes, hipMemcpyDeviceToDevice, N->stream));
-        hipLaunchKernelGGL(k_to_input, dim3(blocks), dim3(NT), 0, N->stream, u, N->x, len,
-                           (float)(1.0 / N->in_scale));
-        int rc = nn_forward(N, (int)B, false);
+        int rc = nn_infer(N, (int)B, u);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_apply_du, dim3(blocks), dim3(NT), 0, N->stream, u, N->du, len, N->out_scale);
-        NN_HIP(hipGetLastError());
         if (!judge) continue;
         // plain solver from the same states: u <- solver^K(usave)
         ++judged;
@@ -923,13 +1194,12 @@ extern "C" int sw1d_surrogate_advance(sw1d_nn* N, sw1d_batch* batch, int64_t sur
             }
             const double err = s > 0 ? sqrt(d / s) : sqrt(d);
             max_err = err > max_err ? err : max_err;
-            if (err <= tol) {  // keep the network's state for this problem
-                NN_HIP(hipMemcpyAsync(u + p * X, N->unn + p * X, X * sizeof(double),
-                                      hipMemcpyDeviceToDevice, N->stream));
-            } else {
-                ++fallbacks;   // the plain solver's state stays
-            }
+            keep[(size_t)p] = err <= tol;  // otherwise the plain solver's state stays
+            fallbacks += err <= tol ? 0 : 1;
         }
+        NN_HIP(hipMemcpyAsync(N->keep, keep.data(), (size_t)B * sizeof(int), hipMemcpyHostToDevice, N->stream));
+        hipLaunchKernelGGL(k_keep_rows, dim3(blocks), dim3(NT), 0, N->stream, u, N->unn, N->keep, (int)X, len);
+        NN_HIP(hipGetLastError());
     }
     NN_HIP(hipStreamSynchronize(N->stream));
     if (stats) {
@@ -1098,7 +1368,7 @@ int sw1d_nn_eval(sw1d_nn* N, int64_t first, int64_t count, double* rel_err)
     for (int64_t s = first; s < first + count; s += N->Bmax) {
         const int b = (int)std::min<int64_t>(N->Bmax, first + count - s);
         NN_HIP(hipMemcpyAsync(N->x, D->x + s * row, b * row * sizeof(float), hipMemcpyDeviceToDevice, N->stream));
-        int rc = nn_forward(N, b, false);
+        int rc = nn_infer(N, b, nullptr);
         if (rc) return rc;
         y.resize(b * row);
         t.resize(b * row);

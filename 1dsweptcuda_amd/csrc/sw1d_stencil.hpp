@@ -3,8 +3,9 @@
 // values as arguments (no array indexing) so that the kernels can feed them
 // from registers.  Operand order follows the reference expressions exactly and
 // all code is compiled with -ffp-contract=off: each operator is one IEEE
-// operation, on the GPU and on the host alike, so every path of this library
-// produces the same bits.
+// operation (the KS stencil's explicit fused multiply-adds one each), on the
+// GPU and on the host alike, so every path of this library produces the same
+// bits.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -24,28 +25,34 @@ SW1D_HD T heat_ftcs(T l, T c, T r, T fo, T fo2)
 
 // -------------------------------------------------------------------- KS ---
 // KS1D_SweptShared.cu:136-155 with the default (multiply by reciprocal)
-// constants; c = {1/(4dx), 1/dx^2, 1/dx^4, dt, dt/2} (KS:917-926).
+// constants; c = {1/(4dx), 1/dx^2, 1/dx^4, dt, dt/2} (KS:917-926):
+//   F = (r^2 - l^2) dx4x + (l + r - 2c) dx2 + (ll - 4l + 6c - 4r + rr) dx4,
+// evaluated with fused multiply-adds at fixed places (one rounding each; the
+// CPU oracle uses the same ones): twelve FP operations per stage.
 template <typename T> struct KsCoef { T dx4x, dx2, dx4, dt, dth; };
+
+SW1D_HD double fmadd(double a, double b, double c) { return __builtin_fma(a, b, c); }
+SW1D_HD float fmadd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
 template <typename T>
 SW1D_HD T ks_rhs(T ll, T l, T c, T r, T rr, const KsCoef<T>& k)
 {
-    const T conv = (r * r - l * l) * k.dx4x;
-    const T sec = (l + r - T(2) * c) * k.dx2;
-    const T fourth = (ll - T(4) * l + T(6) * c - T(4) * r + rr) * k.dx4;
-    return conv + sec + fourth;
+    const T conv = fmadd(r, r, -(l * l));                                   // r^2 - l^2
+    const T sec = fmadd(T(-2), c, l + r);                                   // l + r - 2c
+    const T fourth = fmadd(T(-4), r, fmadd(T(6), c, fmadd(T(-4), l, ll))) + rr;
+    return fmadd(conv, k.dx4x, fmadd(sec, k.dx2, fourth * k.dx4));
 }
 // KS:159-164 (stutterStep): predictor u_half = u - dt/2 * F(u).
 template <typename T>
 SW1D_HD T ks_predict(T ll, T l, T c, T r, T rr, const KsCoef<T>& k)
 {
-    return c - k.dth * ks_rhs(ll, l, c, r, rr, k);
+    return fmadd(-k.dth, ks_rhs(ll, l, c, r, rr, k), c);
 }
 // KS:166-171 (finalStep) and the "+=" of KS:202: u + (-dt * F(u_half)).
 template <typename T>
 SW1D_HD T ks_correct(T u, T ll, T l, T c, T r, T rr, const KsCoef<T>& k)
 {
-    return u + (-k.dt * ks_rhs(ll, l, c, r, rr, k));
+    return fmadd(-k.dt, ks_rhs(ll, l, c, r, rr, k), u);
 }
 
 // ----------------------------------------------------------------- Euler ---

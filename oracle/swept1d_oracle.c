@@ -32,24 +32,37 @@
 #define M_PI 3.14159265358979323846
 #endif
 
+/* The KS stage is built twice, for CPUs with the FMA instructions and for
+ * those without (glibc's correctly rounded fma(), the same results, slower);
+ * the loader picks the clone once. */
+#if defined(__x86_64__)
+#define ORC_FMA_CLONES __attribute__((target_clones("fma", "default")))
+#else
+#define ORC_FMA_CLONES
+#endif
+
 #define REAL double
 #define SFX _d
 #define RSQRT sqrt
+#define RFMA fma
 #define RFABS fabs
 #include "oracle_body.inc"
 #undef REAL
 #undef SFX
 #undef RSQRT
+#undef RFMA
 #undef RFABS
 
 #define REAL float
 #define SFX _f
 #define RSQRT sqrtf
+#define RFMA fmaf
 #define RFABS fabsf
 #include "oracle_body.inc"
 #undef REAL
 #undef SFX
 #undef RSQRT
+#undef RFMA
 #undef RFABS
 
 /* ------------------------------------------------ seeded synthetic ICs --- */
