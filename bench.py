@@ -39,7 +39,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per whole-diamond launch (nX, node width) -> bytes, from rocprofv3
 # FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by its reading for a copy
 # kernel of known size (profiles/ks_swept_diamond_nx1048576_hbm_bytes.md).
-DIAMOND_TRAFFIC_BYTES = {(1 << 20, 512): 33.96e6}
+DIAMOND_TRAFFIC_BYTES = {(1 << 20, 512): 33.96e6, (1 << 20, 1024): 33.84e6}
 
 
 def _timed(plan, steps):
@@ -56,9 +56,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--warmup", type=int, default=2000)
     ap.add_argument("--nx", type=int, default=1 << 20)
-    ap.add_argument("--node-width", type=int, default=512)
+    ap.add_argument("--node-width", type=int, default=1024)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-seconds", type=float, default=16.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the trapezoid engine and the batched workload")
@@ -136,7 +136,10 @@ def main():
                    "nx": n, "dx": 0.5, "dt": dt, "algorithm": "shared swept (alg 1)",
                    "node_width": W, "steps_per_tile_launch": W // 8},
         "classic": {"value": classic_value, "ms_per_step": csecs / csteps * 1e3,
-                    "stage_kernel_us": classic_stage_us, "stage_hbm_gbs": classic_gbs},
+                    "stage_kernel_us": classic_stage_us,
+                    "roofline": {"bound": "hbm", "achieved": classic_gbs, "peak": HBM_PEAK_GBS,
+                                 "unit": "GB/s", "frac": classic_gbs / HBM_PEAK_GBS,
+                                 "algorithmic_bytes_per_pu": 40}},
         "swept_over_classic": value / classic_value,
         "roofline": {"bound": "mfma", "compute": "fp64 vector ALU (no MFMA); peak = MI355X FP64",
                      "kernel": "k_ks_swept<double,%d,DIAMOND>" % (W // 64),
@@ -171,7 +174,7 @@ def _copy_calibration(mib):
 def _trapezoid(sw, n, dt, u0, args):
     """Same workload on the overlapped trapezoid engine (algorithm 3)."""
     res = {}
-    for W, depth in ((512, 1), (1024, 2)):
+    for W, depth in ((512, 2), (1024, 2)):
         p = sw.Plan(sw.KS, n, dt, algorithm=sw.TRAPEZOID, node_width=W, tile_depth=depth)
         p.set_state(u0)
         p.advance(args.warmup)
@@ -261,19 +264,31 @@ def _surrogate():
 
 
 def _cpu_baseline(n, dt, u0, budget_s):
+    """The C oracle (the same KS stages, FP64) on this host: OpenMP over the
+    host threads this job may use (at most 16), and one thread; each on a
+    sample of about budget_s/2 seconds of the same nX."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
 
     c = pyoracle.ks_consts(dt)
-    t0 = time.perf_counter()
-    pyoracle.ks_run(u0, 2, c)
-    per_step = (time.perf_counter() - t0) / 2
-    steps = max(2, int(budget_s / max(per_step, 1e-9)))
-    t0 = time.perf_counter()
-    pyoracle.ks_run(u0, steps, c)
-    secs = time.perf_counter() - t0
-    return {"value": n * steps / secs, "unit": "point-updates/s", "cores": 1, "kind": "port",
-            "sample": f"KS FP64 nX={n}, {steps} steps, serial C oracle (oracle/swept1d_oracle.c)"}
+
+    def rate(threads, seconds):
+        t0 = time.perf_counter()
+        pyoracle.ks_run(u0, 2, c, threads)
+        per_step = (time.perf_counter() - t0) / 2
+        steps = max(2, int(seconds / max(per_step, 1e-9)))
+        t0 = time.perf_counter()
+        pyoracle.ks_run(u0, steps, c, threads)
+        return n * steps / (time.perf_counter() - t0), steps
+
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    threads = max(1, min(16, avail))
+    omp, omp_steps = rate(threads, budget_s / 2)
+    one, one_steps = rate(1, budget_s / 2)
+    return {"value": omp, "unit": "point-updates/s", "cores": threads, "kind": "port",
+            "sample": f"KS FP64 nX={n}, {omp_steps} steps, C oracle (oracle/swept1d_oracle.c) "
+                      f"with OpenMP over {threads} threads",
+            "serial": {"value": one, "cores": 1, "sample": f"same, {one_steps} steps, one thread"}}
 
 
 if __name__ == "__main__":

@@ -43,6 +43,45 @@ def test_forward_matches_oracle(sw):
     assert _rel(y, want) < 3e-3, _rel(y, want)
 
 
+def test_forward_matches_oracle_unfused_width(sw):
+    """nx = 192 is not a multiple of the 256-position tile: the per-layer
+    64-position kernels run instead of the fused ones."""
+    x = np.random.default_rng(5).standard_normal((2, 192)).astype(np.float32)
+    p = _params(6)
+    print("surrogate forward nx=192 B=2", flush=True)
+    with sw.Surrogate(192, 2, seed=0) as nn:
+        nn.set_params(p)
+        y = nn.forward(x)
+    want, _ = no.forward(p.astype(np.float64), x, bf=True)
+    assert _rel(y, want) < 3e-3, _rel(y, want)
+
+
+def test_network_step_is_forward_plus_update(sw):
+    """One unjudged network step of the integrator (fused kernels reading the
+    FP64 state) equals u + out_scale * net(FP32(u) * FP32(1/in_scale)) from
+    the forward entry point, bit for bit."""
+    import importlib
+
+    wl = importlib.import_module("1dsweptcuda_amd.workloads")
+    B, nx = 3, 1024
+    u0 = np.concatenate([wl.fourier_ic(nx, nx * 0.5, 40 + i) for i in range(B)])
+    p = _params(7)
+    in_s, out_s = 1.7, 0.05
+    print("surrogate network step B=3", flush=True)
+    with sw.Surrogate(nx, B, seed=0) as nn, sw.Batch(np.ones(B, np.int32), np.full(B, nx, np.int64),
+                                                      np.full(B, 0.005), np.full(B, 0.5)) as b:
+        nn.set_params(p)
+        nn.set_scales(in_s, out_s)
+        b.set_state(u0)
+        st = nn.advance(b, 1, 20, judge_every=0)
+        assert st["network_steps"] == 1 and st["judged"] == 0
+        u1 = b.get_state()
+        x = (u0.astype(np.float32) * np.float32(1.0 / in_s)).reshape(B, nx)
+        y = nn.forward(x)
+    want = u0 + out_s * y.ravel().astype(np.float64)
+    assert np.array_equal(u1, want), float(np.abs(u1 - want).max())
+
+
 def test_gradient_matches_oracle(sw):
     rng = np.random.default_rng(3)
     x = rng.standard_normal((2, 1024)).astype(np.float32)
