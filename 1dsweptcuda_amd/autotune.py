@@ -59,16 +59,20 @@ def tile_points_for(nx, table: dict, default: int = 256) -> np.ndarray:
     return res
 
 
-def tune_batch(sw, eq, nx, dt, dx, ics, precision: int = 0, steps: int = 400, sample: int = 96,
+def tune_batch(sw, eq, nx, dt, dx, ics, precision: int = 0, steps: int = 400, sample: int = 2048,
                candidates=CANDIDATES, reps: int = 2, seed: int = 0):
-    """Time every candidate per size class on the device; return (table, timings)."""
+    """Time every candidate per size class on the device; return (table, timings).
+
+    Each class is timed as a batch of `sample` of its problems (drawn with
+    repetition when the class has fewer), enough workgroups to fill every CU,
+    so the timing is the class's throughput in a full launch rather than the
+    latency of a few problems."""
     rng = np.random.default_rng(seed)
     eq, nx, dt, dx = map(np.asarray, (eq, nx, dt, dx))
     timings = {}
     for cls in sorted({size_class(n) for n in nx}):
         idx = np.flatnonzero([size_class(n) == cls for n in nx])
-        if idx.size > sample:
-            idx = np.sort(rng.choice(idx, sample, replace=False))
+        idx = np.sort(rng.choice(idx, sample, replace=idx.size < sample))
         pts = float(nx[idx].sum()) * steps
         packed = np.concatenate([ics[i] for i in idx])
         row = {}
