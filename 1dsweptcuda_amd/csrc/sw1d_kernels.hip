@@ -304,9 +304,10 @@ __global__ __launch_bounds__(kClassicBlock) void k_heat_classic(const T* __restr
     out[i] = heat_ftcs(in[l], in[i], in[r], fo, fo2);
 }
 
+// One KS stage, one point per thread (any n).
 template <typename T, int STAGE>
-__global__ __launch_bounds__(kClassicBlock) void k_ks_classic(T* __restrict__ buf, int64_t n,
-                                                              KsCoef<T> k)
+__global__ __launch_bounds__(kClassicBlock) void k_ks_classic1(T* __restrict__ buf, int64_t n,
+                                                               KsCoef<T> k)
 {
     const int64_t i = (int64_t)blockIdx.x * kClassicBlock + threadIdx.x;
     if (i >= n) return;
@@ -317,6 +318,40 @@ __global__ __launch_bounds__(kClassicBlock) void k_ks_classic(T* __restrict__ bu
         buf[n + i] = ks_predict(ll, l, c, r, rr, k);
     else
         buf[i] = ks_correct(buf[i], ll, l, c, r, rr, k);
+}
+
+// One KS stage, two points per thread (n even, so both halves of buf are
+// 16-byte aligned): the six inputs i-2 .. i+3 of points i, i+1 arrive as three
+// aligned pairs, and the two results leave as one pair.  The threads at the
+// ends of the array wrap.
+template <typename T, int STAGE>
+__global__ __launch_bounds__(kClassicBlock) void k_ks_classic(T* __restrict__ buf, int64_t n,
+                                                              KsCoef<T> k)
+{
+    typedef T pair_t __attribute__((ext_vector_type(2)));
+    const int64_t i = 2 * ((int64_t)blockIdx.x * kClassicBlock + threadIdx.x);
+    if (i >= n) return;
+    const T* __restrict__ src = STAGE == 0 ? buf : buf + n;
+    T e[6];
+    if (i >= 2 && i + 4 <= n) {
+        const pair_t* p = reinterpret_cast<const pair_t*>(src + i - 2);
+        const pair_t x0 = p[0], x1 = p[1], x2 = p[2];
+        e[0] = x0.x; e[1] = x0.y; e[2] = x1.x; e[3] = x1.y; e[4] = x2.x; e[5] = x2.y;
+    } else {
+#pragma unroll
+        for (int j = 0; j < 6; ++j) e[j] = src[wrap(i - 2 + j, n)];
+    }
+    pair_t o;
+    if (STAGE == 0) {
+        o.x = ks_predict(e[0], e[1], e[2], e[3], e[4], k);
+        o.y = ks_predict(e[1], e[2], e[3], e[4], e[5], k);
+        *reinterpret_cast<pair_t*>(buf + n + i) = o;
+    } else {
+        const pair_t u = *reinterpret_cast<const pair_t*>(buf + i);
+        o.x = ks_correct(u.x, e[0], e[1], e[2], e[3], e[4], k);
+        o.y = ks_correct(u.y, e[1], e[2], e[3], e[4], e[5], k);
+        *reinterpret_cast<pair_t*>(buf + i) = o;
+    }
 }
 
 template <typename T, int STAGE>
@@ -455,10 +490,17 @@ hipError_t classic_prec(int eq, int bc, int stage, int64_t n, const void* in, vo
                                fo_of<T>(c), fo2_of<T>(c));
     } else if (eq == EQ_KS) {
         T* b = static_cast<T*>(out);
-        if (stage == 0)
-            hipLaunchKernelGGL((k_ks_classic<T, 0>), grid, dim3(kClassicBlock), 0, s, b, n, ks_of<T>(c));
-        else
-            hipLaunchKernelGGL((k_ks_classic<T, 1>), grid, dim3(kClassicBlock), 0, s, b, n, ks_of<T>(c));
+        const dim3 g2((unsigned)((n / 2 + kClassicBlock - 1) / kClassicBlock));
+        if (n % 2 != 0) {
+            if (stage == 0)
+                hipLaunchKernelGGL((k_ks_classic1<T, 0>), grid, dim3(kClassicBlock), 0, s, b, n, ks_of<T>(c));
+            else
+                hipLaunchKernelGGL((k_ks_classic1<T, 1>), grid, dim3(kClassicBlock), 0, s, b, n, ks_of<T>(c));
+        } else if (stage == 0) {
+            hipLaunchKernelGGL((k_ks_classic<T, 0>), g2, dim3(kClassicBlock), 0, s, b, n, ks_of<T>(c));
+        } else {
+            hipLaunchKernelGGL((k_ks_classic<T, 1>), g2, dim3(kClassicBlock), 0, s, b, n, ks_of<T>(c));
+        }
     } else {
         T* b = static_cast<T*>(out);
         if (stage == 0)
