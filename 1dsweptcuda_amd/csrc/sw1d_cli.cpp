@@ -7,17 +7,35 @@
 // (SW1D_CLI_F64 = 1 double, 0 float), like the six reference binaries
 // (SweptSource/Makefile:17-35).
 //
-//   <prog> nX tpb dt tf freq alg solution.dat [timing.txt]
+//   <prog> nX tpb dt tf freq alg solution.dat [timing.txt] [--key=value ...]
 //
 // The reference's time bookkeeping is replayed by sw1d_schedule(); the
 // library advances the state by the resulting step counts.
+//
+// Options (no reference counterpart; without them the programs behave exactly
+// like the reference's):
+//   --ic=reference|fourier|noisy  initial condition: the reference's (default);
+//        heat "fourier" 1 + sum_{k=1..4} a_k sin(2 pi k x/L), a_k ~ U[0,1),
+//        L = nX*ds; KS "noisy" 2cos(19 pi x/128) + N(0, 1e-3)
+//   --seed=S          seed of the fourier / noisy initial condition (default 1;
+//                     splitmix64, so every host draws the same values)
+//   --bc=mirror|periodic  heat boundary (default mirror, the reference's)
+//   --snapshot=FILE   write the final state in full precision (binary)
+//   --resume=FILE     start from a snapshot instead of the initial condition;
+//                     printed times continue from the snapshot's time
+// Snapshot file: the 8 bytes "SW1DSNP1", int32 equation, int32 precision
+// (0 double, 1 float), int64 nX, int64 values per point (1, or 3 for Euler),
+// double time, double dt, then nX * values REALs in the reference's layout.
 
 #include <chrono>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <fstream>
 #include <iostream>
+#include <string>
 #include <vector>
 
 #include "../../include/swept1d.h"
@@ -83,6 +101,110 @@ void invalid_exit()
     exit(-1);
 }
 
+// ------------------------------------------------------------- options ---
+struct Options {
+    string ic = "reference", bc, snapshot, resume;
+    uint64_t seed = 1;
+};
+
+[[noreturn]] void option_exit(const char* arg)
+{
+    cout << "INVALID OPTION: " << arg << endl;
+    exit(-1);
+}
+
+// Removes the --key=value options from argv (in place) and returns them.
+Options take_options(int& argc, char* argv[])
+{
+    Options o;
+    int k = 1;
+    for (int i = 1; i < argc; ++i) {
+        const char* a = argv[i];
+        if (strncmp(a, "--", 2) != 0) {
+            argv[k++] = argv[i];
+            continue;
+        }
+        const char* eq = strchr(a, '=');
+        if (!eq || eq[1] == 0) option_exit(a);
+        const string key(a + 2, eq), val(eq + 1);
+        if (key == "ic") o.ic = val;
+        else if (key == "seed") o.seed = strtoull(val.c_str(), nullptr, 10);
+        else if (key == "bc") o.bc = val;
+        else if (key == "snapshot") o.snapshot = val;
+        else if (key == "resume") o.resume = val;
+        else option_exit(a);
+    }
+    argc = k;
+    return o;
+}
+
+// splitmix64 (Steele, Lea, Flood 2014) and the draws made from it.
+uint64_t splitmix64(uint64_t& s)
+{
+    uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+double uniform01(uint64_t& s) { return (double)(splitmix64(s) >> 11) * (1.0 / 9007199254740992.0); }
+[[maybe_unused]] double normal01(uint64_t& s)  // Box-Muller, cosine branch
+{
+    const double u1 = 1.0 - uniform01(s);
+    const double u2 = uniform01(s);
+    return sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
+}
+
+const char kSnapMagic[8] = {'S', 'W', '1', 'D', 'S', 'N', 'P', '1'};
+constexpr int kSnapPrecision = (sizeof(REAL) == 8) ? 0 : 1;
+
+void write_snapshot(const string& path, int equation, int64_t nx, int64_t comps, double t, double dt,
+                    const vector<REAL>& v)
+{
+    ofstream f(path, ios::binary | ios::trunc);
+    const int32_t eq32 = equation, pr32 = kSnapPrecision;
+    f.write(kSnapMagic, 8);
+    f.write(reinterpret_cast<const char*>(&eq32), 4);
+    f.write(reinterpret_cast<const char*>(&pr32), 4);
+    f.write(reinterpret_cast<const char*>(&nx), 8);
+    f.write(reinterpret_cast<const char*>(&comps), 8);
+    f.write(reinterpret_cast<const char*>(&t), 8);
+    f.write(reinterpret_cast<const char*>(&dt), 8);
+    f.write(reinterpret_cast<const char*>(v.data()), (streamsize)(v.size() * sizeof(REAL)));
+    if (!f) {
+        cout << "cannot write snapshot " << path << endl;
+        exit(-1);
+    }
+}
+
+// Reads a snapshot of this program's equation, precision and size; returns its time.
+double read_snapshot(const string& path, int equation, int64_t nx, int64_t comps, vector<REAL>& v)
+{
+    ifstream f(path, ios::binary);
+    char magic[8];
+    int32_t eq32 = -1, pr32 = -1;
+    int64_t n = 0, c = 0;
+    double t = 0, dt = 0;
+    f.read(magic, 8);
+    f.read(reinterpret_cast<char*>(&eq32), 4);
+    f.read(reinterpret_cast<char*>(&pr32), 4);
+    f.read(reinterpret_cast<char*>(&n), 8);
+    f.read(reinterpret_cast<char*>(&c), 8);
+    f.read(reinterpret_cast<char*>(&t), 8);
+    f.read(reinterpret_cast<char*>(&dt), 8);
+    if (!f || memcmp(magic, kSnapMagic, 8) != 0 || eq32 != equation || pr32 != kSnapPrecision || n != nx ||
+        c != comps) {
+        cout << "snapshot " << path << " does not hold this program's equation, precision and nX" << endl;
+        exit(-1);
+    }
+    v.resize((size_t)(nx * comps));
+    f.read(reinterpret_cast<char*>(v.data()), (streamsize)(v.size() * sizeof(REAL)));
+    if (!f) {
+        cout << "snapshot " << path << " is truncated" << endl;
+        exit(-1);
+    }
+    return t;
+}
+
 #if SW1D_CLI_EQ == 2
 // Euler:752-759 (energy) and :183-193 (pressure) in REAL arithmetic.
 REAL energy_of(const REAL* c)
@@ -114,6 +236,7 @@ void euler_rows(ofstream& f, double t, const vector<REAL>& q, int dv, REAL mgam,
 
 int main(int argc, char* argv[])
 {
+    const Options opt = take_options(argc, argv);
     if (argc < 8) usage_exit();
     cout.precision(10);
 
@@ -150,10 +273,28 @@ int main(int argc, char* argv[])
     if ((dv & ((tpb - 1) != 0)) || (tpb & 31) != 0) invalid_exit();  // Heat:735 as evaluated
     IC.resize((size_t)dv);
     const REAL lxr = (REAL)lx;
-    for (int k = 0; k < dv; k++) {
-        const REAL a = ((REAL)k * ds);
-        IC[k] = REAL(100.f) * a * (REAL(1) - a / lxr);
+    if (opt.ic == "fourier") {
+        // periodic workload: 1 + sum_{k=1..4} a_k sin(2 pi k x / L), x = j L / nX
+        uint64_t st = opt.seed;
+        double amp[4];
+        for (double& a : amp) a = uniform01(st);
+        const double L = (double)dv * (double)ds;
+        for (int j = 0; j < dv; j++) {
+            const double x = (double)j * (L / (double)dv);
+            double v = 1.0;
+            for (int k = 0; k < 4; k++) v += amp[k] * sin(2.0 * M_PI * (double)(k + 1) * x / L);
+            IC[j] = (REAL)v;
+        }
+    } else if (opt.ic == "reference") {
+        for (int k = 0; k < dv; k++) {
+            const REAL a = ((REAL)k * ds);
+            IC[k] = REAL(100.f) * a * (REAL(1) - a / lxr);
+        }
+    } else {
+        option_exit(("--ic=" + opt.ic).c_str());
     }
+    if (opt.bc == "periodic") prob.bc = SW1D_BC_PERIODIC;
+    else if (!opt.bc.empty() && opt.bc != "mirror") option_exit(("--bc=" + opt.bc).c_str());
     sched_kind = scheme ? 1 : 0;
     row_tag = scheme ? "Temperature " : " Temperature ";
 #elif SW1D_CLI_EQ == 1
@@ -165,10 +306,22 @@ int main(int argc, char* argv[])
          << " | dt/dx: " << dt / dx << endl;
     if ((dv & ((tpb - 1) != 0)) || (tpb & 31) != 0) invalid_exit();  // KS:897 as evaluated
     IC.resize((size_t)dv);
-    for (int k = 0; k < dv; k++) {
-        const REAL x = (REAL)k * (REAL)dx;
-        IC[k] = (REAL)(2.0 * cos(19.0 * (double)x * M_PI / 128.0));
+    if (opt.ic == "noisy") {
+        // KS workload: reference IC plus N(0, 1e-3) noise per point, x = k dx
+        uint64_t st = opt.seed;
+        for (int k = 0; k < dv; k++) {
+            const double x = (double)k * (double)dx;
+            IC[k] = (REAL)(2.0 * cos(19.0 * x * M_PI / 128.0) + 1e-3 * normal01(st));
+        }
+    } else if (opt.ic == "reference") {
+        for (int k = 0; k < dv; k++) {
+            const REAL x = (REAL)k * (REAL)dx;
+            IC[k] = (REAL)(2.0 * cos(19.0 * (double)x * M_PI / 128.0));
+        }
+    } else {
+        option_exit(("--ic=" + opt.ic).c_str());
     }
+    if (!opt.bc.empty() && opt.bc != "periodic") option_exit(("--bc=" + opt.bc).c_str());
     sched_kind = scheme == 0 ? 2 : (scheme == 1 ? 3 : 4);
     dt_sched = (double)(REAL)dt;  // the wrappers receive dsc.dt (KS:975, 979)
     row_tag = " Velocity ";
@@ -188,6 +341,8 @@ int main(int argc, char* argv[])
         cout << "The value of dt/dx (" << dt_dx << ") is too high.  In general it must be <=.21 for stability." << endl;
         exit(-1);
     }
+    if (opt.ic != "reference") option_exit(("--ic=" + opt.ic).c_str());
+    if (!opt.bc.empty()) option_exit(("--bc=" + opt.bc).c_str());
     const REAL bl[3] = {REAL(1), REAL(0), REAL(1) / mgam};
     const REAL br[3] = {(REAL)0.125, REAL(0), (REAL)(0.1 / (double)mgam)};
     IC.resize((size_t)dv * 3);
@@ -200,20 +355,23 @@ int main(int argc, char* argv[])
         exit(-1);
     }
 
+    const int64_t comps = SW1D_CLI_EQ == 2 ? 3 : 1;
+    const double t_start = opt.resume.empty() ? 0.0 : read_snapshot(opt.resume, SW1D_CLI_EQ, dv, comps, IC);
+
     ofstream fwr;
     fwr.open(argv[7], ios::trunc);
     fwr.precision(10);
 #if SW1D_CLI_EQ == 0
-    fwr << lx << " " << dv << " " << ds << " " << endl << "Temperature " << 0 << " ";
+    fwr << lx << " " << dv << " " << ds << " " << endl << "Temperature " << t_start << " ";
     for (int k = 0; k < dv; k++) fwr << IC[k] << " ";
     fwr << endl;
 #elif SW1D_CLI_EQ == 1
-    fwr << lx << " " << dv << " " << dx << " " << endl << " Velocity " << 0 << " ";
+    fwr << lx << " " << dv << " " << dx << " " << endl << " Velocity " << t_start << " ";
     for (int k = 0; k < dv; k++) fwr << IC[k] << " ";
     fwr << endl;
 #else
     fwr << lxr << " " << (dv - 2) << " " << dx << " " << endl;
-    euler_rows(fwr, 0, IC, dv, mgam, true);
+    euler_rows(fwr, t_start, IC, dv, mgam, t_start == 0.0);
 #endif
 
 #if SW1D_CLI_EQ == 1
@@ -275,7 +433,7 @@ int main(int argc, char* argv[])
         if (sw1d_advance(plan, rows.steps[i] - done) != SW1D_OK) die_lib("advance");
         done = rows.steps[i];
         if (sw1d_get_state(plan, Tf.data()) != SW1D_OK) die_lib("get_state");
-        const double t = rows.times[i];
+        const double t = t_start + rows.times[i];
 #if SW1D_CLI_EQ == 2
         euler_rows(fwr, t, Tf, dv, mgam, false);
 #else
@@ -315,15 +473,16 @@ int main(int argc, char* argv[])
     }
 
 #if SW1D_CLI_EQ == 0
-    fwr << "Temperature " << tfm << " ";
+    fwr << "Temperature " << t_start + tfm << " ";
     for (int k = 0; k < dv; k++) fwr << Tf[k] << " ";
 #elif SW1D_CLI_EQ == 1
-    fwr << " Velocity " << tfm << " ";
+    fwr << " Velocity " << t_start + tfm << " ";
     for (int k = 0; k < dv; k++) fwr << Tf[k] << " ";
     fwr << endl;
 #else
-    euler_rows(fwr, tfm, Tf, dv, mgam, false);
+    euler_rows(fwr, t_start + tfm, Tf, dv, mgam, false);
 #endif
     fwr.close();
+    if (!opt.snapshot.empty()) write_snapshot(opt.snapshot, SW1D_CLI_EQ, dv, comps, t_start + tfm, dt, Tf);
     return 0;
 }

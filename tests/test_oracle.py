@@ -175,56 +175,12 @@ def test_ks_single_precision_tracks_double(orc):
 GAM = 1.4
 
 
-def sod_exact(x, t, x0=0.5, L=(1.0, 0.0, 1.0), R=(0.125, 0.0, 0.1), g=GAM):
-    """Exact Riemann solution (Toro, Riemann Solvers and Numerical Methods for
-    Fluid Dynamics, ch. 4): Newton on the pressure function, then sampling."""
-    rl, ul, pl = L
-    rr, ur, pr = R
-    al, ar = np.sqrt(g * pl / rl), np.sqrt(g * pr / rr)
+def sod_exact(x, t):
+    """Toro's exact Sod solution, from the analysis module (pinned below by the
+    published star state)."""
+    import importlib
 
-    def fk(p, rk, pk, ak):
-        if p > pk:
-            A, B = 2 / ((g + 1) * rk), (g - 1) / (g + 1) * pk
-            f = (p - pk) * np.sqrt(A / (p + B))
-            df = np.sqrt(A / (B + p)) * (1 - (p - pk) / (2 * (B + p)))
-        else:
-            f = 2 * ak / (g - 1) * ((p / pk) ** ((g - 1) / (2 * g)) - 1)
-            df = 1 / (rk * ak) * (p / pk) ** (-(g + 1) / (2 * g))
-        return f, df
-
-    p = 0.5 * (pl + pr)
-    for _ in range(60):
-        f1, d1 = fk(p, rl, pl, al)
-        f2, d2 = fk(p, rr, pr, ar)
-        dp = (f1 + f2 + ur - ul) / (d1 + d2)
-        p = max(p - dp, 1e-12)
-        if abs(dp) < 1e-15 * p:
-            break
-    f1, _ = fk(p, rl, pl, al)
-    f2, _ = fk(p, rr, pr, ar)
-    us = 0.5 * (ul + ur) + 0.5 * (f2 - f1)
-    rsl = rl * (p / pl) ** (1 / g)  # left rarefaction (p* < pL for Sod)
-    rsr = rr * ((p / pr + (g - 1) / (g + 1)) / ((g - 1) / (g + 1) * p / pr + 1))
-    out = np.zeros((4, x.size))
-    xi = (x - x0) / t
-    shl, stl = ul - al, us - al * (p / pl) ** ((g - 1) / (2 * g))
-    sr = ur + ar * np.sqrt((g + 1) / (2 * g) * p / pr + (g - 1) / (2 * g))
-    for i, s in enumerate(xi):
-        if s < shl:
-            r, u, pp = rl, ul, pl
-        elif s < stl:
-            u = 2 / (g + 1) * (al + (g - 1) / 2 * ul + s)
-            a = 2 / (g + 1) * (al + (g - 1) / 2 * (ul - s))
-            r = rl * (a / al) ** (2 / (g - 1))
-            pp = pl * (a / al) ** (2 * g / (g - 1))
-        elif s < us:
-            r, u, pp = rsl, us, p
-        elif s < sr:
-            r, u, pp = rsr, us, p
-        else:
-            r, u, pp = rr, ur, pr
-        out[:, i] = (r, u, pp / ((g - 1) * r), pp)
-    return out, (p, us, rsl, rsr)
+    return importlib.import_module("1dsweptcuda_amd.harness").sod_exact(x, t)
 
 
 def test_sod_exact_star_state():
