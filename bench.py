@@ -147,6 +147,7 @@ def main():
     }
     if not args.no_extras:
         out["trapezoid"] = _trapezoid(sw, n, dt, rng_ic, args)
+        out["equations"] = _equations(sw, n)
         out["batched"] = _batched(sw)
         out["surrogate"] = _surrogate()
     if not args.no_cpu:
@@ -164,6 +165,38 @@ def _copy_calibration(mib):
     torch.neg(a, out=b)
     torch.cuda.synchronize()
     del a, b
+
+
+def _equations(sw, n):
+    """Heat (the reference's parabola, insulated ends, Fo = 0.084) and Euler
+    (Sod, dt/dx = 0.1) at the same nX, FP64: classic, shared swept and
+    trapezoid point-updates/s (every engine gives the oracle's bits)."""
+    res = {}
+    x = np.arange(n) * 0.001
+    lx = 0.001 * (n - 1)
+    heat_ic = 100.0 * x * (1.0 - x / lx)
+    q = np.zeros((n, 3))
+    q[: n // 2] = (1.0, 0.0, 1.0 / 0.4)
+    q[n // 2:] = (0.125, 0.0, 0.1 / 0.4)
+    dt_eu = 0.1 / (n - 2)
+    cases = {
+        "heat": (sw.HEAT, 1e-3, heat_ic, 20000, ((sw.CLASSIC, 256, 0), (sw.SWEPT, 512, 0), (sw.TRAPEZOID, 256, 1))),
+        "euler": (sw.EULER, dt_eu, q.ravel(), 2000, ((sw.CLASSIC, 256, 0), (sw.SWEPT, 256, 0), (sw.TRAPEZOID, 256, 1))),
+    }
+    for name, (eq, dt, ic, steps, algs) in cases.items():
+        row = {}
+        for alg, W, depth in algs:
+            p = sw.Plan(eq, n, dt, algorithm=alg, tpb=W, node_width=W, tile_depth=depth)
+            p.set_state(ic)
+            k = steps if alg != sw.CLASSIC else max(steps // 5, 100)
+            p.advance(k // 10)
+            secs = _timed(p, k)
+            p.close()
+            row[{sw.CLASSIC: "classic", sw.SWEPT: "swept", sw.TRAPEZOID: "trapezoid"}[alg]] = n * k / secs
+        row["swept_over_classic"] = row["swept"] / row["classic"]
+        res[name] = row
+    res["unit"] = "point-updates/s"
+    return res
 
 
 def _trapezoid(sw, n, dt, u0, args):
