@@ -80,6 +80,22 @@ def test_parity_vs_oracle(sw, orc, eq, bc, prec, alg):
     assert bad.size == 0, f"{bad.size} mismatches, first at {bad[:8]}: {got[bad[:4]]} vs {want[bad[:4]]}"
 
 
+@pytest.mark.parametrize("W", [512, 1024])
+def test_ks_phased_diamond_single_precision(sw, orc, W):
+    """The KS diamonds with P >= 8 points per lane run in four region widths
+    from an LDS image of the node; FP32 through every launch kind."""
+    n = 8 * W
+    dt = _dt(1, n, 1)
+    nsteps = 5 * (W // 8) + 3
+    _say(f"phased KS FP32 W={W} n={n} steps={nsteps}")
+    u0 = _ic(orc, 1, n, np.float32, 1)
+    with sw.Plan(1, n, dt, precision=1, algorithm=1, node_width=W) as p:
+        p.set_state(u0)
+        p.advance(nsteps)
+        got = p.get_state()
+    assert np.array_equal(got, _oracle(orc, 1, u0, nsteps, dt, np.float32, 0, n))
+
+
 @pytest.mark.parametrize("eq", [0, 1, 2])
 @pytest.mark.parametrize("W", [64, 128, 256, 512, 1024])
 def test_node_widths(sw, orc, eq, W):
