@@ -88,7 +88,105 @@ __device__ __forceinline__ int64_t wrap(int64_t i, int64_t n)
 // the reference's working array, Heat:151-164): a point just inside a
 // triangle edge is one level ahead of its outer neighbour, and the neighbour
 // needs the level before.  Odd stages write row 1 from row 0, even stages
-// row 0 from row 1.
+// row 0 from row 1.  HeatRegion holds PC points per lane of the middle
+// 64*PC points of a node (all of it when PC = P).
+template <typename T, int PC, bool MIRROR>
+struct HeatRegion {
+    T r0[PC], r1[PC];
+    T hl0[1], hr0[1], hl1[1], hr1[1];
+    int d[PC];
+    bool g0, gN;  // this lane's first / last point is global point 0 / n-1
+
+    __device__ __forceinline__ void stage(int thr, const T (&src)[PC], const T (&hl)[1], const T (&hr)[1],
+                                          T (&dst)[PC], T fo, T fo2)
+    {
+        T e[PC + 2];
+        gather<T, PC, 1>(src, hl, hr, e);
+        // Insulated ends (Heat:140-142): the left neighbour of point 0 is
+        // point 1, the right neighbour of point n-1 is point n-2.
+        if (MIRROR) {
+            e[0] = g0 ? e[2] : e[0];
+            e[PC + 1] = gN ? e[PC - 1] : e[PC + 1];
+        }
+#pragma unroll
+        for (int p = 0; p < PC; ++p) {
+            const T nv = heat_ftcs(e[p], e[p + 1], e[p + 2], fo, fo2);
+            dst[p] = (d[p] <= thr) ? nv : dst[p];
+        }
+    }
+    // down pairs s = s0, s0+2, .. s1 (thresholds 2s-1, 2s+1); up pairs
+    // m = m0, .. m1 (thresholds W-1-2m, W-3-2m)
+    __device__ __forceinline__ void down(int s0, int s1, T fo, T fo2)
+    {
+        for (int s = s0; s <= s1; s += 2) {
+            stage(2 * s - 1, r0, hl0, hr0, r1, fo, fo2);
+            stage(2 * s + 1, r1, hl1, hr1, r0, fo, fo2);
+        }
+    }
+    __device__ __forceinline__ void up(int W, int m0, int m1, T fo, T fo2)
+    {
+        for (int m = m0; m <= m1; m += 2) {
+            stage(W - 1 - 2 * m, r0, hl0, hr0, r1, fo, fo2);
+            stage(W - 3 - 2 * m, r1, hl1, hr1, r0, fo, fo2);
+        }
+    }
+    // flags and distance codes for a lane whose first point is node-local x0
+    __device__ __forceinline__ void place(int W, int x0, int64_t base, int64_t n)
+    {
+#pragma unroll
+        for (int p = 0; p < PC; ++p) d[p] = abs(2 * (x0 + p) - (W - 1));
+        int64_t gi = base + x0;
+        gi = gi >= n ? gi - n : gi;
+        int64_t gl = gi + PC - 1;
+        gl = gl >= n ? gl - n : gl;
+        g0 = MIRROR && gi == 0;
+        gN = MIRROR && gl == n - 1;
+    }
+    template <int W>
+    __device__ __forceinline__ void load(const T* img, int lane, int64_t base, int64_t n)
+    {
+        constexpr int xs = W / 2 - 32 * PC;
+        static_assert(xs >= 1 && xs + 64 * PC + 1 <= W, "region and halo inside the node");
+        const int x0 = xs + lane * PC;
+#pragma unroll
+        for (int p = 0; p < PC; ++p) {
+            r0[p] = img[x0 + p];
+            r1[p] = img[W + x0 + p];
+        }
+        hl0[0] = img[xs - 1];
+        hr0[0] = img[xs + 64 * PC];
+        hl1[0] = img[W + xs - 1];
+        hr1[0] = img[W + xs + 64 * PC];
+        place(W, x0, base, n);
+    }
+    template <int W>
+    __device__ __forceinline__ void save(T* img, int lane) const
+    {
+        const int x0 = W / 2 - 32 * PC + lane * PC;
+#pragma unroll
+        for (int p = 0; p < PC; ++p) {
+            img[x0 + p] = r0[p];
+            img[W + x0 + p] = r1[p];
+        }
+    }
+};
+
+// Orders a wave's own LDS writes before its own reads of other lanes' data:
+// LDS executes a wave's instructions in order, and the wavefront-scope
+// fences keep the compiler from moving one lane's loads above other lanes'
+// stores.  Each wave of the tile kernels owns its slice of the LDS image.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Swept heat tiles: one wave per node of W = 64*P points.  For P >= 4 the
+// node's image sits in LDS and the diamond's levels run in four region
+// widths (the middle quarter, half, three quarters, all of the node), each
+// level on the narrowest region holding its active set and the one-point
+// stencil reach: the same operations, so the same bits, on fewer points.
 template <typename T, int P, int TILE, bool MIRROR>
 __global__ __launch_bounds__(kBlock) void k_heat_swept(const T* __restrict__ in,
                                                        T* __restrict__ out, int64_t n,
@@ -96,50 +194,100 @@ __global__ __launch_bounds__(kBlock) void k_heat_swept(const T* __restrict__ in,
                                                        int64_t offset, T fo, T fo2)
 {
     constexpr int W = kWave * P, H = W / 2;
+    constexpr bool PHASED = P >= 4 && P % 4 == 0;
+    __shared__ T img_all[PHASED ? kWavesPerBlock * 2 * W : 1];
     NodeGeom g;
     if (!node_geom<P>(n, nnodes, node_first, offset, g)) return;
 
     const T* __restrict__ in1 = in + n;
-    T r0[P], r1[P];
-    load_pts<T, P>(in, g.lb, r0);
-    load_pts<T, P>(in1, g.lb, r1);
-    T hl0[1] = {in[wrap(g.base - 1, n)]}, hr0[1] = {in[wrap(g.base + W, n)]};
-    T hl1[1] = {in1[wrap(g.base - 1, n)]}, hr1[1] = {in1[wrap(g.base + W, n)]};
-    int d[P];
-#pragma unroll
-    for (int p = 0; p < P; ++p) d[p] = abs(2 * (g.lane * P + p) - (W - 1));
-    // Insulated ends (Heat:140-142): the left neighbour of point 0 is point 1,
-    // the right neighbour of point n-1 is point n-2.
-    const bool g0 = MIRROR && g.lb == 0;
-    const bool gN = MIRROR && g.lb + P - 1 == n - 1;
+    HeatRegion<T, P, MIRROR> full;
+    load_pts<T, P>(in, g.lb, full.r0);
+    load_pts<T, P>(in1, g.lb, full.r1);
+    full.hl0[0] = in[wrap(g.base - 1, n)];
+    full.hr0[0] = in[wrap(g.base + W, n)];
+    full.hl1[0] = in1[wrap(g.base - 1, n)];
+    full.hr1[0] = in1[wrap(g.base + W, n)];
+    full.place(W, g.lane * P, g.base, n);
 
-    auto stage = [&](int thr, const T (&src)[P], const T (&hl)[1], const T (&hr)[1], T (&dst)[P]) {
-        T e[P + 2];
-        gather<T, P, 1>(src, hl, hr, e);
-        if (MIRROR) {
-            e[0] = g0 ? e[2] : e[0];
-            e[P + 1] = gN ? e[P - 1] : e[P + 1];
+    if constexpr (!PHASED) {
+        if (TILE != TILE_UP) full.down(1, H - 1, fo, fo2);
+        if (TILE != TILE_DOWN) {
+            full.up(W, 1, H - 3, fo, fo2);
+            full.stage(W - 1 - 2 * (H - 1), full.r0, full.hl0, full.hr0, full.r1, fo, fo2);
         }
+    } else {
+        // region widths w_j = j W/4 take down pairs s <= 32 j Q - 1 and up
+        // pairs m >= 32 (4 - j) Q + 1 (every threshold <= w_j - 1)
+        constexpr int Q = P / 4;
+        constexpr int S1 = 32 * Q - 1, S2 = 64 * Q - 1, S3 = 96 * Q - 1;
+        constexpr int M3 = 32 * Q + 1, M2 = 64 * Q + 1, M1 = 96 * Q + 1;
+        T* img = img_all + (threadIdx.x >> 6) * 2 * W;
+        const int lane = g.lane;
+        auto reload_full = [&] {
 #pragma unroll
-        for (int p = 0; p < P; ++p) {
-            const T nv = heat_ftcs(e[p], e[p + 1], e[p + 2], fo, fo2);
-            dst[p] = (d[p] <= thr) ? nv : dst[p];
+            for (int p = 0; p < P; ++p) {
+                full.r0[p] = img[lane * P + p];
+                full.r1[p] = img[W + lane * P + p];
+            }
+        };
+        if (TILE != TILE_UP) {
+            full.template save<W>(img, lane);
+            wave_sync();
+            {
+                HeatRegion<T, Q, MIRROR> r;
+                r.template load<W>(img, lane, g.base, n);
+                r.down(1, S1, fo, fo2);
+                r.template save<W>(img, lane);
+                wave_sync();
+            }
+            {
+                HeatRegion<T, 2 * Q, MIRROR> r;
+                r.template load<W>(img, lane, g.base, n);
+                r.down(S1 + 2, S2, fo, fo2);
+                r.template save<W>(img, lane);
+                wave_sync();
+            }
+            {
+                HeatRegion<T, 3 * Q, MIRROR> r;
+                r.template load<W>(img, lane, g.base, n);
+                r.down(S2 + 2, S3, fo, fo2);
+                r.template save<W>(img, lane);
+                wave_sync();
+            }
+            reload_full();
+            full.down(S3 + 2, H - 1, fo, fo2);
         }
-    };
-    if (TILE != TILE_UP)
-        for (int s = 1; s <= H; s += 2) {
-            stage(2 * s - 1, r0, hl0, hr0, r1);
-            stage(2 * s + 1, r1, hl1, hr1, r0);
+        if (TILE != TILE_DOWN) {
+            full.up(W, 1, M3 - 2, fo, fo2);
+            full.template save<W>(img, lane);
+            wave_sync();
+            {
+                HeatRegion<T, 3 * Q, MIRROR> r;
+                r.template load<W>(img, lane, g.base, n);
+                r.up(W, M3, M2 - 2, fo, fo2);
+                r.template save<W>(img, lane);
+                wave_sync();
+            }
+            {
+                HeatRegion<T, 2 * Q, MIRROR> r;
+                r.template load<W>(img, lane, g.base, n);
+                r.up(W, M2, M1 - 2, fo, fo2);
+                r.template save<W>(img, lane);
+                wave_sync();
+            }
+            {
+                HeatRegion<T, Q, MIRROR> r;
+                r.template load<W>(img, lane, g.base, n);
+                r.up(W, M1, H - 3, fo, fo2);
+                r.stage(W - 1 - 2 * (H - 1), r.r0, r.hl0, r.hr0, r.r1, fo, fo2);
+                r.template save<W>(img, lane);
+                wave_sync();
+            }
+            reload_full();
         }
-    if (TILE != TILE_DOWN) {
-        for (int m = 1; m < H - 1; m += 2) {
-            stage(W - 1 - 2 * m, r0, hl0, hr0, r1);
-            stage(W - 3 - 2 * m, r1, hl1, hr1, r0);
-        }
-        stage(W - 1 - 2 * (H - 1), r0, hl0, hr0, r1);
     }
-    store_pts<T, P>(out, g.lb, r0);
-    store_pts<T, P>(out + n, g.lb, r1);
+    store_pts<T, P>(out, g.lb, full.r0);
+    store_pts<T, P>(out + n, g.lb, full.r1);
 }
 
 // ------------------------------------------------------------- KS tiles ---
@@ -266,17 +414,8 @@ __global__ __launch_bounds__(kBlock) void k_ks_swept(const T* __restrict__ in,
         constexpr int Q = P / 4;
         constexpr int S1 = 16 * Q - 1, S2 = 32 * Q - 1, S3 = 48 * Q - 1;
         constexpr int M3 = 16 * Q + 1, M2 = 32 * Q + 1, M1 = 48 * Q + 1;
-        // Each wave owns its slice of the LDS image, so ordering its own
-        // writes before its own reads is all that is needed: LDS executes a
-        // wave's instructions in order, and the fences (wavefront scope) keep
-        // the compiler from moving one lane's loads above other lanes' stores.
         T* img = img_all + (threadIdx.x >> 6) * 2 * W;
         const int lane = g.lane;
-        auto wave_sync = [] {
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        };
         auto to_image = [&](const auto& r) {
             r.template save<W>(img, lane);
             wave_sync();

@@ -97,3 +97,30 @@ def test_ks_region_phases_contain_active_sets(P):
             assert act.min() >= lo and act.max() < hi, (pc, t)
             assert act.min() - 2 >= lo - 2 and act.max() + 2 <= hi + 1
             assert np.all(d[halo] > t), (pc, t)
+
+
+@pytest.mark.parametrize("P", [4, 8, 16])
+def test_heat_region_phases_contain_active_sets(P):
+    """Same check for the heat diamond kernel (k_heat_swept, P >= 4): one-point
+    stencil, thresholds 2s-1, 2s+1 down and W-1-2m, W-3-2m up."""
+    W, H, Q = 64 * P, 32 * P, P // 4
+    S1, S2, S3 = 32 * Q - 1, 64 * Q - 1, 96 * Q - 1
+    M3, M2, M1 = 32 * Q + 1, 64 * Q + 1, 96 * Q + 1
+    down = lambda s0, s1: [t for s in range(s0, s1 + 1, 2) for t in (2 * s - 1, 2 * s + 1)]
+    up = lambda m0, m1: [t for m in range(m0, m1 + 1, 2) for t in (W - 1 - 2 * m, W - 3 - 2 * m)]
+    phases = [(Q, down(1, S1)), (2 * Q, down(S1 + 2, S2)), (3 * Q, down(S2 + 2, S3)),
+              (P, down(S3 + 2, H - 1) + up(1, M3 - 2)), (3 * Q, up(M3, M2 - 2)), (2 * Q, up(M2, M1 - 2)),
+              (Q, up(M1, H - 3) + [W - 1 - 2 * (H - 1)])]
+    reference = down(1, H - 1) + up(1, H - 3) + [W - 1 - 2 * (H - 1)]
+    assert [t for _, ts in phases for t in ts] == reference
+    x = np.arange(W)
+    d = np.abs(2 * x - (W - 1))
+    for pc, ts in phases:
+        lo, hi = W // 2 - 32 * pc, W // 2 + 32 * pc
+        if pc < P:
+            assert lo >= 1 and hi + 1 <= W
+        halo = np.array([lo - 1, hi]) if pc < P else np.array([], int)
+        for t in ts:
+            act = x[d <= t]
+            assert act.min() >= lo and act.max() < hi, (pc, t)
+            assert np.all(d[halo] > t), (pc, t)
