@@ -484,72 +484,54 @@ __global__ __launch_bounds__(kBlock) void k_ks_swept(const T* __restrict__ in,
 }
 
 // ---------------------------------------------------------- Euler tiles ---
-template <typename T, int P, int TILE>
-__global__ __launch_bounds__(kBlock) void k_euler_swept(const T* __restrict__ in,
-                                                        T* __restrict__ out, int64_t n,
-                                                        int64_t nnodes, int64_t node_first,
-                                                        int64_t offset, EulerCoef<T> k)
-{
-    constexpr int W = kWave * P, H = W / 4;
-    NodeGeom g;
-    if (!node_geom<P>(n, nnodes, node_first, offset, g)) return;
-
-    // Components: 0..2 conserved (rho, rho*u, E), 3..5 their predictor.
-    T q[6][P];
+// Components 0..2 are the conserved values (rho, rho*u, E), 3..5 their
+// predictor.  EulerRegion holds PC cells per lane of the middle 64*PC cells
+// of a node (all of it when PC = P).
+template <typename T, int PC>
+struct EulerRegion {
+    T q[6][PC];
     T hl[6][2], hr[6][2];
-#pragma unroll
-    for (int c = 0; c < 6; ++c) {
-        load_pts<T, P>(in + c * n, g.lb, q[c]);
-#pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            hl[c][i] = in[c * n + wrap(g.base - 1 - i, n)];
-            hr[c][i] = in[c * n + wrap(g.base + W + i, n)];
-        }
-    }
-    int d[P];
-#pragma unroll
-    for (int p = 0; p < P; ++p) d[p] = abs(2 * (g.lane * P + p) - (W - 1));
-    // Ghost cells 0 and n-1 hold the Dirichlet states; cells 1 and n-2 drop
-    // the outer pressure difference (Euler:405-425).
-    const bool g0 = g.lb == 0;
-    const bool gN = g.lb + P - 1 == n - 1;
+    int d[PC];
+    bool g0, gN;  // this lane's first / last cell is global cell 0 / n-1
 
     // src = 0: predictor from the conserved values (Euler:307-350);
     // src = 3: corrector from the predictor values (Euler:353-389, :420).
     // Each interface flux is evaluated once and used by the two cells beside
-    // it (P+1 interfaces per lane instead of 2P).
-    auto stage = [&](int thr, auto src_tag) {
-        constexpr int src = decltype(src_tag)::value;
+    // it (PC+1 interfaces per lane instead of 2 PC).
+    template <int src>
+    __device__ __forceinline__ void stage(int thr, const EulerCoef<T>& k)
+    {
         constexpr int dst = 3 - src;
-        T ex[P + 4], ey[P + 4], ez[P + 4];
-        gather<T, P, 2>(q[src + 0], hl[src + 0], hr[src + 0], ex);
-        gather<T, P, 2>(q[src + 1], hl[src + 1], hr[src + 1], ey);
-        gather<T, P, 2>(q[src + 2], hl[src + 2], hr[src + 2], ez);
+        T ex[PC + 4], ey[PC + 4], ez[PC + 4];
+        gather<T, PC, 2>(q[src + 0], hl[src + 0], hr[src + 0], ex);
+        gather<T, PC, 2>(q[src + 1], hl[src + 1], hr[src + 1], ey);
+        gather<T, PC, 2>(q[src + 2], hl[src + 2], hr[src + 2], ez);
         auto cell = [&](int j) { return Q3<T>{ex[j], ey[j], ez[j]}; };
         // pressure differences between ext cells j and j+1
-        T pd[P + 3];
+        T pd[PC + 3];
 #pragma unroll
-        for (int j = 0; j < P + 3; ++j) pd[j] = eu_pdiff(cell(j), cell(j + 1));
-        // interface j lies between ext cells j+1 and j+2 (cells lb+j-1, lb+j);
-        // the interface left of global cell 1 drops pdiff(-1,0), the one right
-        // of cell n-2 drops pdiff(n-1,n) (the flags of Euler:405, :424).
+        for (int j = 0; j < PC + 3; ++j) pd[j] = eu_pdiff(cell(j), cell(j + 1));
+        // interface j lies between ext cells j+1 and j+2; the interface left
+        // of global cell 1 drops pdiff(-1,0), the one right of cell n-2 drops
+        // pdiff(n-1,n) (the flags of Euler:405, :424).
         auto iface = [&](int j, Q3<T>& F, Q3<T>& D) {
             const T q0 = (j == 1 && g0) ? T(0) : pd[j];
-            const T q2 = (j == P - 1 && gN) ? T(0) : pd[j + 2];
+            const T q2 = (j == PC - 1 && gN) ? T(0) : pd[j + 2];
             eu_iface(cell(j), cell(j + 1), cell(j + 2), cell(j + 3), q0, pd[j + 1], q2, k, F, D);
         };
         const T c = src == 0 ? T(0.25) * k.dtdx : T(0.5) * k.dtdx;
         Q3<T> FL, DL;
         iface(0, FL, DL);
 #pragma unroll
-        for (int p = 0; p < P; ++p) {
+        for (int p = 0; p < PC; ++p) {
             Q3<T> FR, DR;
             iface(p + 1, FR, DR);
             const Q3<T> f = eu_balance(FL, DL, FR, DR);
             const Q3<T> base = src == 0 ? cell(p + 2) : Q3<T>{q[0][p], q[1][p], q[2][p]};
             Q3<T> nv = eu_apply(base, f, c);
+            // ghost cells 0 and n-1 hold the Dirichlet states (Euler:405-425)
             if (p == 0 && g0) nv = k.bl;
-            if (p == P - 1 && gN) nv = k.br;
+            if (p == PC - 1 && gN) nv = k.br;
             const bool act = d[p] <= thr;
             q[dst + 0][p] = act ? nv.x : q[dst + 0][p];
             q[dst + 1][p] = act ? nv.y : q[dst + 1][p];
@@ -557,17 +539,144 @@ __global__ __launch_bounds__(kBlock) void k_euler_swept(const T* __restrict__ in
             FL = FR;
             DL = DR;
         }
-    };
-    using A = std::integral_constant<int, 0>;
-    using B = std::integral_constant<int, 3>;
-    if (TILE != TILE_UP)
-        for (int s = 1; s <= H; s += 2) { stage(4 * s - 1, A{}); stage(4 * s + 3, B{}); }
-    if (TILE != TILE_DOWN) {
-        for (int m = 1; m < H - 1; m += 2) { stage(W - 1 - 4 * m, A{}); stage(W - 5 - 4 * m, B{}); }
-        stage(W - 1 - 4 * (H - 1), A{});
+    }
+    __device__ __forceinline__ void down(int s0, int s1, const EulerCoef<T>& k)
+    {
+        for (int s = s0; s <= s1; s += 2) { stage<0>(4 * s - 1, k); stage<3>(4 * s + 3, k); }
+    }
+    __device__ __forceinline__ void up(int W, int m0, int m1, const EulerCoef<T>& k)
+    {
+        for (int m = m0; m <= m1; m += 2) { stage<0>(W - 1 - 4 * m, k); stage<3>(W - 5 - 4 * m, k); }
+    }
+    __device__ __forceinline__ void place(int W, int x0, int64_t base, int64_t n)
+    {
+#pragma unroll
+        for (int p = 0; p < PC; ++p) d[p] = abs(2 * (x0 + p) - (W - 1));
+        int64_t gi = base + x0;
+        gi = gi >= n ? gi - n : gi;
+        int64_t gl = gi + PC - 1;
+        gl = gl >= n ? gl - n : gl;
+        g0 = gi == 0;
+        gN = gl == n - 1;
+    }
+    template <int W>
+    __device__ __forceinline__ void load(const T* img, int lane, int64_t base, int64_t n)
+    {
+        constexpr int xs = W / 2 - 32 * PC;
+        static_assert(xs >= 2 && xs + 64 * PC + 2 <= W, "region and halo inside the node");
+        const int x0 = xs + lane * PC;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+#pragma unroll
+            for (int p = 0; p < PC; ++p) q[c][p] = img[c * W + x0 + p];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                hl[c][i] = img[c * W + xs - 1 - i];
+                hr[c][i] = img[c * W + xs + 64 * PC + i];
+            }
+        }
+        place(W, x0, base, n);
+    }
+    template <int W>
+    __device__ __forceinline__ void save(T* img, int lane) const
+    {
+        const int x0 = W / 2 - 32 * PC + lane * PC;
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+#pragma unroll
+            for (int p = 0; p < PC; ++p) img[c * W + x0 + p] = q[c][p];
+    }
+};
+
+// Swept Euler tiles: one wave per node of W = 64*P cells.  With the node's
+// image in LDS the diamond's levels run on the narrowest centred region that
+// holds their active set and the two-cell stencil reach: widths W/4 .. W for
+// P = 8, W/2 and W for P = 4.  Same operations, same bits.
+template <typename T, int P, int TILE>
+__global__ __launch_bounds__(kBlock) void k_euler_swept(const T* __restrict__ in,
+                                                        T* __restrict__ out, int64_t n,
+                                                        int64_t nnodes, int64_t node_first,
+                                                        int64_t offset, EulerCoef<T> k)
+{
+    constexpr int W = kWave * P, H = W / 4;
+    constexpr int NPH = P == 8 ? 4 : (P == 4 ? 2 : 1);  // region widths
+    __shared__ T img_all[NPH > 1 ? kWavesPerBlock * 6 * W : 1];
+    NodeGeom g;
+    if (!node_geom<P>(n, nnodes, node_first, offset, g)) return;
+
+    EulerRegion<T, P> full;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        load_pts<T, P>(in + c * n, g.lb, full.q[c]);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            full.hl[c][i] = in[c * n + wrap(g.base - 1 - i, n)];
+            full.hr[c][i] = in[c * n + wrap(g.base + W + i, n)];
+        }
+    }
+    full.place(W, g.lane * P, g.base, n);
+
+    if constexpr (NPH == 1) {
+        if (TILE != TILE_UP) full.down(1, H - 1, k);
+        if (TILE != TILE_DOWN) {
+            full.up(W, 1, H - 3, k);
+            full.template stage<0>(W - 1 - 4 * (H - 1), k);
+        }
+    } else {
+        // region j of NPH (j < NPH) has PC = j P / NPH cells per lane and takes
+        // down pairs s <= 16 PC - 1 and up pairs m >= 16 (P - PC) + 1
+        T* img = img_all + (threadIdx.x >> 6) * 6 * W;
+        const int lane = g.lane;
+        auto region = [&](auto pc_tag, auto&& body) {
+            constexpr int PC = decltype(pc_tag)::value;
+            EulerRegion<T, PC> r;
+            r.template load<W>(img, lane, g.base, n);
+            body(r);
+            r.template save<W>(img, lane);
+            wave_sync();
+        };
+        auto reload_full = [&] {
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+#pragma unroll
+                for (int p = 0; p < P; ++p) full.q[c][p] = img[c * W + lane * P + p];
+        };
+        using PC1 = std::integral_constant<int, P / NPH>;
+        using PC2 = std::integral_constant<int, 2 * P / NPH>;
+        using PC3 = std::integral_constant<int, 3 * P / NPH>;
+        constexpr int S1 = 16 * PC1::value - 1, S2 = 16 * PC2::value - 1, S3 = 16 * PC3::value - 1;
+        constexpr int M1 = 16 * (P - PC1::value) + 1, M2 = 16 * (P - PC2::value) + 1,
+                      M3 = 16 * (P - PC3::value) + 1;
+        constexpr int SL = NPH == 4 ? S3 : S1;  // last region pair before the full width
+        constexpr int ML = NPH == 4 ? M3 : M1;  // first region pair after it
+        if (TILE != TILE_UP) {
+            full.template save<W>(img, lane);
+            wave_sync();
+            region(PC1{}, [&](auto& r) { r.down(1, S1, k); });
+            if constexpr (NPH == 4) {
+                region(PC2{}, [&](auto& r) { r.down(S1 + 2, S2, k); });
+                region(PC3{}, [&](auto& r) { r.down(S2 + 2, S3, k); });
+            }
+            reload_full();
+            full.down(SL + 2, H - 1, k);
+        }
+        if (TILE != TILE_DOWN) {
+            full.up(W, 1, ML - 2, k);
+            full.template save<W>(img, lane);
+            wave_sync();
+            if constexpr (NPH == 4) {
+                region(PC3{}, [&](auto& r) { r.up(W, M3, M2 - 2, k); });
+                region(PC2{}, [&](auto& r) { r.up(W, M2, M1 - 2, k); });
+            }
+            region(PC1{}, [&](auto& r) {
+                r.up(W, M1, H - 3, k);
+                r.template stage<0>(W - 1 - 4 * (H - 1), k);
+            });
+            reload_full();
+        }
     }
 #pragma unroll
-    for (int c = 0; c < 6; ++c) store_pts<T, P>(out + c * n, g.lb, q[c]);
+    for (int c = 0; c < 6; ++c) store_pts<T, P>(out + c * n, g.lb, full.q[c]);
 }
 
 // ------------------------------------------------------ classic kernels ---

@@ -124,3 +124,35 @@ def test_heat_region_phases_contain_active_sets(P):
             act = x[d <= t]
             assert act.min() >= lo and act.max() < hi, (pc, t)
             assert np.all(d[halo] > t), (pc, t)
+
+
+@pytest.mark.parametrize("P,pcs", [(4, (2,)), (8, (2, 4, 6))])
+def test_euler_region_phases_contain_active_sets(P, pcs):
+    """k_euler_swept: two-cell stencil like KS; regions of PC = P/2 (P = 4) or
+    P/4, P/2, 3P/4 (P = 8) cells per lane below the full width."""
+    W, H = 64 * P, 16 * P
+    S = [16 * pc - 1 for pc in pcs]
+    M = [16 * (P - pc) + 1 for pc in pcs]
+    down = lambda s0, s1: [t for s in range(s0, s1 + 1, 2) for t in (4 * s - 1, 4 * s + 3)]
+    up = lambda m0, m1: [t for m in range(m0, m1 + 1, 2) for t in (W - 1 - 4 * m, W - 5 - 4 * m)]
+    phases, s0 = [], 1
+    for pc, s1 in zip(pcs, S):
+        phases.append((pc, down(s0, s1)))
+        s0 = s1 + 2
+    phases.append((P, down(s0, H - 1) + up(1, M[-1] - 2)))
+    for i in range(len(pcs) - 1, 0, -1):
+        phases.append((pcs[i], up(M[i], M[i - 1] - 2)))
+    phases.append((pcs[0], up(M[0], H - 3) + [W - 1 - 4 * (H - 1)]))
+    reference = down(1, H - 1) + up(1, H - 3) + [W - 1 - 4 * (H - 1)]
+    assert [t for _, ts in phases for t in ts] == reference
+    x = np.arange(W)
+    d = np.abs(2 * x - (W - 1))
+    for pc, ts in phases:
+        lo, hi = W // 2 - 32 * pc, W // 2 + 32 * pc
+        if pc < P:
+            assert lo >= 2 and hi + 2 <= W
+        halo = np.array([lo - 2, lo - 1, hi, hi + 1]) if pc < P else np.array([], int)
+        for t in ts:
+            act = x[d <= t]
+            assert act.min() >= lo and act.max() < hi, (pc, t)
+            assert np.all(d[halo] > t), (pc, t)
