@@ -290,6 +290,13 @@ __global__ __launch_bounds__(kBlock) void k_heat_swept(const T* __restrict__ in,
     store_pts<T, P>(out + n, g.lb, full.r1);
 }
 
+// Calls f(integral_constant<int, i>) for i = 0, 1, .. in order.
+template <typename F, int... I>
+__device__ __forceinline__ void for_each_index(F&& f, std::integer_sequence<int, I...>)
+{
+    (f(std::integral_constant<int, I>{}), ...);
+}
+
 // ------------------------------------------------------------- KS tiles ---
 // One RK2 stage pair on PC points per lane.  e[] holds the lane's points with
 // two neighbours on each side (DPP from the adjacent lanes, hl / hr at the
@@ -372,11 +379,11 @@ struct KsRegion {
 // Swept KS tiles: one wave per node of W = 64*P points, lane l owning points
 // l*P .. l*P+P-1 in VGPRs.  A whole diamond runs W/2 stage levels but its
 // active set is W wide only in the middle; for P >= 8 the node's image is
-// kept in LDS and the levels run in four widths (the middle quarter, half,
-// three quarters, all of the node: P/4 .. P points per lane), each level on
-// the narrowest region that holds its active set and the two-point stencil
-// reach.  Same operations on the same values as the one-width form, so the
-// same bits, with 37% fewer point evaluations.
+// kept in LDS and the levels run on centred regions of 2, 4, .. P points per
+// lane, each level on the narrowest region that holds its active set and the
+// two-point stencil reach.  Same operations on the same values as the
+// one-width form, so the same bits, with 44% fewer point evaluations at
+// P = 16 (37% at P = 8).
 template <typename T, int P, int TILE>
 __global__ __launch_bounds__(kBlock) void k_ks_swept(const T* __restrict__ in,
                                                      T* __restrict__ out, int64_t n,
@@ -409,74 +416,55 @@ __global__ __launch_bounds__(kBlock) void k_ks_swept(const T* __restrict__ in,
             full.stage_a(W - 1 - 4 * (H - 1), k);
         }
     } else {
-        // Region widths w_j = j*W/4 (j = 1..3) take down pairs s <= 16 j P/4 - 1
-        // (both thresholds <= w_j - 1) and up pairs m >= 16 (P - jP/4) + 1.
-        constexpr int Q = P / 4;
-        constexpr int S1 = 16 * Q - 1, S2 = 32 * Q - 1, S3 = 48 * Q - 1;
-        constexpr int M3 = 16 * Q + 1, M2 = 32 * Q + 1, M1 = 48 * Q + 1;
+        // Regions of PC = 2, 4, .. P-2 points per lane (widths 128 PC): region
+        // PC takes down pairs 16(PC-2)+1 <= s <= 16 PC - 1 and up pairs
+        // 16(P-PC)+1 <= m <= 16(P-PC)+31, where every threshold is <= 64 PC - 1.
         T* img = img_all + (threadIdx.x >> 6) * 2 * W;
         const int lane = g.lane;
-        auto to_image = [&](const auto& r) {
+        auto reload_full = [&] {
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                full.u[p] = img[lane * P + p];
+                full.uh[p] = img[W + lane * P + p];
+            }
+        };
+        auto down_region = [&](auto pc_tag) {
+            constexpr int PC = decltype(pc_tag)::value;
+            KsRegion<T, PC> r;
+            r.template load<W>(img, lane);
+            r.down(PC == 2 ? 1 : 16 * PC - 31, 16 * PC - 1, k);
             r.template save<W>(img, lane);
             wave_sync();
         };
-        auto from_image = [&](auto& r) { r.template load<W>(img, lane); };
-        if (TILE != TILE_UP) {
-            full.template save<W>(img, lane);  // the whole node: region of P points
+        auto up_region = [&](auto pc_tag) {
+            constexpr int PC = decltype(pc_tag)::value;
+            KsRegion<T, PC> r;
+            r.template load<W>(img, lane);
+            if constexpr (PC == 2) {
+                r.up(W, 16 * (P - 2) + 1, H - 3, k);
+                r.stage_a(W - 1 - 4 * (H - 1), k);
+            } else {
+                r.up(W, 16 * (P - PC) + 1, 16 * (P - PC) + 31, k);
+            }
+            r.template save<W>(img, lane);
             wave_sync();
-            {
-                KsRegion<T, Q> r;
-                from_image(r);
-                r.down(1, S1, k);
-                to_image(r);
-            }
-            {
-                KsRegion<T, 2 * Q> r;
-                from_image(r);
-                r.down(S1 + 2, S2, k);
-                to_image(r);
-            }
-            {
-                KsRegion<T, 3 * Q> r;
-                from_image(r);
-                r.down(S2 + 2, S3, k);
-                to_image(r);
-            }
-#pragma unroll
-            for (int p = 0; p < P; ++p) {
-                full.u[p] = img[lane * P + p];
-                full.uh[p] = img[W + lane * P + p];
-            }
-            full.down(S3 + 2, H - 1, k);
-        }
-        if (TILE != TILE_DOWN) {
-            full.up(W, 1, M3 - 2, k);
+        };
+        using Regions = std::make_integer_sequence<int, P / 2 - 1>;  // i -> PC = 2 (i + 1)
+        if (TILE != TILE_UP) {
             full.template save<W>(img, lane);
             wave_sync();
-            {
-                KsRegion<T, 3 * Q> r;
-                from_image(r);
-                r.up(W, M3, M2 - 2, k);
-                to_image(r);
-            }
-            {
-                KsRegion<T, 2 * Q> r;
-                from_image(r);
-                r.up(W, M2, M1 - 2, k);
-                to_image(r);
-            }
-            {
-                KsRegion<T, Q> r;
-                from_image(r);
-                r.up(W, M1, H - 3, k);
-                r.stage_a(W - 1 - 4 * (H - 1), k);
-                to_image(r);
-            }
-#pragma unroll
-            for (int p = 0; p < P; ++p) {
-                full.u[p] = img[lane * P + p];
-                full.uh[p] = img[W + lane * P + p];
-            }
+            for_each_index([&](auto i) { down_region(std::integral_constant<int, 2 * (decltype(i)::value + 1)>{}); },
+                           Regions{});
+            reload_full();
+            full.down(16 * (P - 2) + 1, H - 1, k);
+        }
+        if (TILE != TILE_DOWN) {
+            full.up(W, 1, 31, k);
+            full.template save<W>(img, lane);
+            wave_sync();
+            for_each_index([&](auto i) { up_region(std::integral_constant<int, P - 2 * (decltype(i)::value + 1)>{}); },
+                           Regions{});
+            reload_full();
         }
     }
     store_pts<T, P>(out, g.lb, full.u);

@@ -180,8 +180,8 @@ def _equations(sw, n):
     q[n // 2:] = (0.125, 0.0, 0.1 / 0.4)
     dt_eu = 0.1 / (n - 2)
     cases = {
-        "heat": (sw.HEAT, 1e-3, heat_ic, 20000, ((sw.CLASSIC, 256, 0), (sw.SWEPT, 512, 0), (sw.TRAPEZOID, 256, 1))),
-        "euler": (sw.EULER, dt_eu, q.ravel(), 2000, ((sw.CLASSIC, 256, 0), (sw.SWEPT, 256, 0), (sw.TRAPEZOID, 256, 1))),
+        "heat": (sw.HEAT, 1e-3, heat_ic, 20000, ((sw.CLASSIC, 256, 0), (sw.SWEPT, 1024, 0), (sw.TRAPEZOID, 256, 1))),
+        "euler": (sw.EULER, dt_eu, q.ravel(), 2000, ((sw.CLASSIC, 256, 0), (sw.SWEPT, 512, 0), (sw.TRAPEZOID, 256, 1))),
     }
     for name, (eq, dt, ic, steps, algs) in cases.items():
         row = {}

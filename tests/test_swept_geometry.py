@@ -68,35 +68,10 @@ def test_emulator_detects_a_wrong_active_set(orc, monkeypatch):
 
 @pytest.mark.parametrize("P", [8, 16])
 def test_ks_region_phases_contain_active_sets(P):
-    """The KS diamond kernel (k_ks_swept, P >= 8) runs its stage levels on the
-    middle quarter / half / three quarters / all of the node (P/4 .. P points
-    per lane).  Its phase boundaries must give back the one-width stage
-    sequence in order, every active point and its two-point stencil reach must
-    lie inside the region plus the two-point halos, and the halo points must
-    stay inactive (constant) for the whole phase."""
-    W, H, Q = 64 * P, 16 * P, P // 4
-    S1, S2, S3 = 16 * Q - 1, 32 * Q - 1, 48 * Q - 1
-    M3, M2, M1 = 16 * Q + 1, 32 * Q + 1, 48 * Q + 1
-    down = lambda s0, s1: [t for s in range(s0, s1 + 1, 2) for t in (4 * s - 1, 4 * s + 3)]
-    up = lambda m0, m1: [t for m in range(m0, m1 + 1, 2) for t in (W - 1 - 4 * m, W - 5 - 4 * m)]
-    phases = [(Q, down(1, S1)), (2 * Q, down(S1 + 2, S2)), (3 * Q, down(S2 + 2, S3)),
-              (P, down(S3 + 2, H - 1) + up(1, M3 - 2)), (3 * Q, up(M3, M2 - 2)), (2 * Q, up(M2, M1 - 2)),
-              (Q, up(M1, H - 3) + [W - 1 - 4 * (H - 1)])]
-    reference = down(1, H - 1) + up(1, H - 3) + [W - 1 - 4 * (H - 1)]
-    assert [t for _, ts in phases for t in ts] == reference
-    x = np.arange(W)
-    d = np.abs(2 * x - (W - 1))
-    for pc, ts in phases:
-        xs = W // 2 - 32 * pc
-        lo, hi = xs, xs + 64 * pc  # region [lo, hi)
-        if pc < P:
-            assert xs >= 2 and hi + 2 <= W  # halos inside the LDS image
-        halo = np.array([lo - 2, lo - 1, hi, hi + 1]) if pc < P else np.array([], int)
-        for t in ts:
-            act = x[d <= t]
-            assert act.min() >= lo and act.max() < hi, (pc, t)
-            assert act.min() - 2 >= lo - 2 and act.max() + 2 <= hi + 1
-            assert np.all(d[halo] > t), (pc, t)
+    """The KS diamond kernel (k_ks_swept, P >= 8) runs its stage levels on
+    centred regions of 2, 4, .. P-2 points per lane below the full width; the
+    same thresholds as Euler's two-cell stencil."""
+    test_euler_region_phases_contain_active_sets(P, tuple(range(2, P, 2)))
 
 
 @pytest.mark.parametrize("P", [4, 8, 16])
