@@ -379,11 +379,11 @@ struct KsRegion {
 // Swept KS tiles: one wave per node of W = 64*P points, lane l owning points
 // l*P .. l*P+P-1 in VGPRs.  A whole diamond runs W/2 stage levels but its
 // active set is W wide only in the middle; for P >= 8 the node's image is
-// kept in LDS and the levels run on centred regions of 2, 4, .. P points per
+// kept in LDS and the levels run on centred regions of 2, 3, .. P points per
 // lane, each level on the narrowest region that holds its active set and the
 // two-point stencil reach.  Same operations on the same values as the
-// one-width form, so the same bits, with 44% fewer point evaluations at
-// P = 16 (37% at P = 8).
+// one-width form, so the same bits, with 46% fewer point evaluations at
+// P = 16 (42% at P = 8).
 template <typename T, int P, int TILE>
 __global__ __launch_bounds__(kBlock) void k_ks_swept(const T* __restrict__ in,
                                                      T* __restrict__ out, int64_t n,
@@ -416,9 +416,10 @@ __global__ __launch_bounds__(kBlock) void k_ks_swept(const T* __restrict__ in,
             full.stage_a(W - 1 - 4 * (H - 1), k);
         }
     } else {
-        // Regions of PC = 2, 4, .. P-2 points per lane (widths 128 PC): region
-        // PC takes down pairs 16(PC-2)+1 <= s <= 16 PC - 1 and up pairs
-        // 16(P-PC)+1 <= m <= 16(P-PC)+31, where every threshold is <= 64 PC - 1.
+        // Regions of PC = 2, 3, .. P-1 points per lane (widths 64 PC): region
+        // PC takes down pairs 16 PC - 15 <= s <= 16 PC - 1 (from 1 for PC = 2)
+        // and up pairs 16(P-PC)+1 <= m <= 16(P-PC)+15 (to H-3 for PC = 2),
+        // where every threshold is <= 64 PC - 1.
         T* img = img_all + (threadIdx.x >> 6) * 2 * W;
         const int lane = g.lane;
         auto reload_full = [&] {
@@ -432,7 +433,7 @@ __global__ __launch_bounds__(kBlock) void k_ks_swept(const T* __restrict__ in,
             constexpr int PC = decltype(pc_tag)::value;
             KsRegion<T, PC> r;
             r.template load<W>(img, lane);
-            r.down(PC == 2 ? 1 : 16 * PC - 31, 16 * PC - 1, k);
+            r.down(PC == 2 ? 1 : 16 * PC - 15, 16 * PC - 1, k);
             r.template save<W>(img, lane);
             wave_sync();
         };
@@ -444,25 +445,25 @@ __global__ __launch_bounds__(kBlock) void k_ks_swept(const T* __restrict__ in,
                 r.up(W, 16 * (P - 2) + 1, H - 3, k);
                 r.stage_a(W - 1 - 4 * (H - 1), k);
             } else {
-                r.up(W, 16 * (P - PC) + 1, 16 * (P - PC) + 31, k);
+                r.up(W, 16 * (P - PC) + 1, 16 * (P - PC) + 15, k);
             }
             r.template save<W>(img, lane);
             wave_sync();
         };
-        using Regions = std::make_integer_sequence<int, P / 2 - 1>;  // i -> PC = 2 (i + 1)
+        using Regions = std::make_integer_sequence<int, P - 2>;  // i -> PC = i + 2
         if (TILE != TILE_UP) {
             full.template save<W>(img, lane);
             wave_sync();
-            for_each_index([&](auto i) { down_region(std::integral_constant<int, 2 * (decltype(i)::value + 1)>{}); },
+            for_each_index([&](auto i) { down_region(std::integral_constant<int, decltype(i)::value + 2>{}); },
                            Regions{});
             reload_full();
-            full.down(16 * (P - 2) + 1, H - 1, k);
+            full.down(16 * (P - 1) + 1, H - 1, k);
         }
         if (TILE != TILE_DOWN) {
-            full.up(W, 1, 31, k);
+            full.up(W, 1, 15, k);
             full.template save<W>(img, lane);
             wave_sync();
-            for_each_index([&](auto i) { up_region(std::integral_constant<int, P - 2 * (decltype(i)::value + 1)>{}); },
+            for_each_index([&](auto i) { up_region(std::integral_constant<int, P - 1 - decltype(i)::value>{}); },
                            Regions{});
             reload_full();
         }
