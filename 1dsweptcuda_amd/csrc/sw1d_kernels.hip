@@ -83,6 +83,13 @@ __device__ __forceinline__ int64_t wrap(int64_t i, int64_t n)
     return i < 0 ? i + n : (i >= n ? i - n : i);
 }
 
+// Calls f(integral_constant<int, i>) for i = 0, 1, .. in order.
+template <typename F, int... I>
+__device__ __forceinline__ void for_each_index(F&& f, std::integer_sequence<int, I...>)
+{
+    (f(std::integral_constant<int, I>{}), ...);
+}
+
 // ----------------------------------------------------------- heat tiles ---
 // Two values per point, the even-level and the odd-level row (the two rows of
 // the reference's working array, Heat:151-164): a point just inside a
@@ -183,10 +190,10 @@ __device__ __forceinline__ void wave_sync()
 }
 
 // Swept heat tiles: one wave per node of W = 64*P points.  For P >= 4 the
-// node's image sits in LDS and the diamond's levels run in four region
-// widths (the middle quarter, half, three quarters, all of the node), each
-// level on the narrowest region holding its active set and the one-point
-// stencil reach: the same operations, so the same bits, on fewer points.
+// node's image sits in LDS and the diamond's levels run on centred regions of
+// 1, 2, 3 (P = 4) or 2, 4, .. P (P >= 8) points per lane, each level on the
+// narrowest region holding its active set and the one-point stencil reach:
+// the same operations, so the same bits, on fewer points.
 template <typename T, int P, int TILE, bool MIRROR>
 __global__ __launch_bounds__(kBlock) void k_heat_swept(const T* __restrict__ in,
                                                        T* __restrict__ out, int64_t n,
@@ -216,11 +223,11 @@ __global__ __launch_bounds__(kBlock) void k_heat_swept(const T* __restrict__ in,
             full.stage(W - 1 - 2 * (H - 1), full.r0, full.hl0, full.hr0, full.r1, fo, fo2);
         }
     } else {
-        // region widths w_j = j W/4 take down pairs s <= 32 j Q - 1 and up
-        // pairs m >= 32 (4 - j) Q + 1 (every threshold <= w_j - 1)
-        constexpr int Q = P / 4;
-        constexpr int S1 = 32 * Q - 1, S2 = 64 * Q - 1, S3 = 96 * Q - 1;
-        constexpr int M3 = 32 * Q + 1, M2 = 64 * Q + 1, M1 = 96 * Q + 1;
+        // Regions of PC = STEP, 2 STEP, .. P - STEP points per lane (widths
+        // 64 PC): region PC takes down pairs 32(PC-STEP)+1 <= s <= 32 PC - 1 and
+        // up pairs 32(P-PC)+1 <= m <= 32(P-PC+STEP)-1 (to H-3 for the
+        // narrowest), where every threshold is <= 64 PC - 1.
+        constexpr int STEP = P >= 8 ? 2 : 1;
         T* img = img_all + (threadIdx.x >> 6) * 2 * W;
         const int lane = g.lane;
         auto reload_full = [&] {
@@ -230,71 +237,47 @@ __global__ __launch_bounds__(kBlock) void k_heat_swept(const T* __restrict__ in,
                 full.r1[p] = img[W + lane * P + p];
             }
         };
+        auto down_region = [&](auto pc_tag) {
+            constexpr int PC = decltype(pc_tag)::value;
+            HeatRegion<T, PC, MIRROR> r;
+            r.template load<W>(img, lane, g.base, n);
+            r.down(32 * (PC - STEP) + 1, 32 * PC - 1, fo, fo2);
+            r.template save<W>(img, lane);
+            wave_sync();
+        };
+        auto up_region = [&](auto pc_tag) {
+            constexpr int PC = decltype(pc_tag)::value;
+            HeatRegion<T, PC, MIRROR> r;
+            r.template load<W>(img, lane, g.base, n);
+            if constexpr (PC == STEP) {
+                r.up(W, 32 * (P - PC) + 1, H - 3, fo, fo2);
+                r.stage(W - 1 - 2 * (H - 1), r.r0, r.hl0, r.hr0, r.r1, fo, fo2);
+            } else {
+                r.up(W, 32 * (P - PC) + 1, 32 * (P - PC + STEP) - 1, fo, fo2);
+            }
+            r.template save<W>(img, lane);
+            wave_sync();
+        };
+        using Regions = std::make_integer_sequence<int, P / STEP - 1>;  // i -> PC = STEP (i + 1)
         if (TILE != TILE_UP) {
             full.template save<W>(img, lane);
             wave_sync();
-            {
-                HeatRegion<T, Q, MIRROR> r;
-                r.template load<W>(img, lane, g.base, n);
-                r.down(1, S1, fo, fo2);
-                r.template save<W>(img, lane);
-                wave_sync();
-            }
-            {
-                HeatRegion<T, 2 * Q, MIRROR> r;
-                r.template load<W>(img, lane, g.base, n);
-                r.down(S1 + 2, S2, fo, fo2);
-                r.template save<W>(img, lane);
-                wave_sync();
-            }
-            {
-                HeatRegion<T, 3 * Q, MIRROR> r;
-                r.template load<W>(img, lane, g.base, n);
-                r.down(S2 + 2, S3, fo, fo2);
-                r.template save<W>(img, lane);
-                wave_sync();
-            }
+            for_each_index([&](auto i) { down_region(std::integral_constant<int, STEP * (decltype(i)::value + 1)>{}); },
+                           Regions{});
             reload_full();
-            full.down(S3 + 2, H - 1, fo, fo2);
+            full.down(32 * (P - STEP) + 1, H - 1, fo, fo2);
         }
         if (TILE != TILE_DOWN) {
-            full.up(W, 1, M3 - 2, fo, fo2);
+            full.up(W, 1, 32 * STEP - 1, fo, fo2);
             full.template save<W>(img, lane);
             wave_sync();
-            {
-                HeatRegion<T, 3 * Q, MIRROR> r;
-                r.template load<W>(img, lane, g.base, n);
-                r.up(W, M3, M2 - 2, fo, fo2);
-                r.template save<W>(img, lane);
-                wave_sync();
-            }
-            {
-                HeatRegion<T, 2 * Q, MIRROR> r;
-                r.template load<W>(img, lane, g.base, n);
-                r.up(W, M2, M1 - 2, fo, fo2);
-                r.template save<W>(img, lane);
-                wave_sync();
-            }
-            {
-                HeatRegion<T, Q, MIRROR> r;
-                r.template load<W>(img, lane, g.base, n);
-                r.up(W, M1, H - 3, fo, fo2);
-                r.stage(W - 1 - 2 * (H - 1), r.r0, r.hl0, r.hr0, r.r1, fo, fo2);
-                r.template save<W>(img, lane);
-                wave_sync();
-            }
+            for_each_index([&](auto i) { up_region(std::integral_constant<int, P - STEP * (decltype(i)::value + 1)>{}); },
+                           Regions{});
             reload_full();
         }
     }
     store_pts<T, P>(out, g.lb, full.r0);
     store_pts<T, P>(out + n, g.lb, full.r1);
-}
-
-// Calls f(integral_constant<int, i>) for i = 0, 1, .. in order.
-template <typename F, int... I>
-__device__ __forceinline__ void for_each_index(F&& f, std::integer_sequence<int, I...>)
-{
-    (f(std::integral_constant<int, I>{}), ...);
 }
 
 // ------------------------------------------------------------- KS tiles ---

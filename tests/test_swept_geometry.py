@@ -77,15 +77,20 @@ def test_ks_region_phases_contain_active_sets(P):
 @pytest.mark.parametrize("P", [4, 8, 16])
 def test_heat_region_phases_contain_active_sets(P):
     """Same check for the heat diamond kernel (k_heat_swept, P >= 4): one-point
-    stencil, thresholds 2s-1, 2s+1 down and W-1-2m, W-3-2m up."""
-    W, H, Q = 64 * P, 32 * P, P // 4
-    S1, S2, S3 = 32 * Q - 1, 64 * Q - 1, 96 * Q - 1
-    M3, M2, M1 = 32 * Q + 1, 64 * Q + 1, 96 * Q + 1
+    stencil, thresholds 2s-1, 2s+1 down and W-1-2m, W-3-2m up; regions of
+    1, 2, 3 (P = 4) or 2, 4, .. P-2 (P >= 8) points per lane."""
+    W, H = 64 * P, 32 * P
+    step = 2 if P >= 8 else 1
+    pcs = list(range(step, P, step))
     down = lambda s0, s1: [t for s in range(s0, s1 + 1, 2) for t in (2 * s - 1, 2 * s + 1)]
     up = lambda m0, m1: [t for m in range(m0, m1 + 1, 2) for t in (W - 1 - 2 * m, W - 3 - 2 * m)]
-    phases = [(Q, down(1, S1)), (2 * Q, down(S1 + 2, S2)), (3 * Q, down(S2 + 2, S3)),
-              (P, down(S3 + 2, H - 1) + up(1, M3 - 2)), (3 * Q, up(M3, M2 - 2)), (2 * Q, up(M2, M1 - 2)),
-              (Q, up(M1, H - 3) + [W - 1 - 2 * (H - 1)])]
+    phases = [(pc, down(32 * (pc - step) + 1, 32 * pc - 1)) for pc in pcs]
+    phases.append((P, down(32 * (P - step) + 1, H - 1) + up(1, 32 * step - 1)))
+    for pc in reversed(pcs):
+        if pc == step:
+            phases.append((pc, up(32 * (P - pc) + 1, H - 3) + [W - 1 - 2 * (H - 1)]))
+        else:
+            phases.append((pc, up(32 * (P - pc) + 1, 32 * (P - pc + step) - 1)))
     reference = down(1, H - 1) + up(1, H - 3) + [W - 1 - 2 * (H - 1)]
     assert [t for _, ts in phases for t in ts] == reference
     x = np.arange(W)
