@@ -560,10 +560,10 @@ struct EulerRegion {
     }
 };
 
-// Swept Euler tiles: one wave per node of W = 64*P cells.  With the node's
-// image in LDS the diamond's levels run on the narrowest centred region that
-// holds their active set and the two-cell stencil reach: widths W/4 .. W for
-// P = 8, W/2 and W for P = 4.  Same operations, same bits.
+// Swept Euler tiles: one wave per node of W = 64*P cells.  For P >= 4 the
+// node's image sits in LDS and the diamond's levels run on the narrowest
+// centred region of 2, 3, .. P cells per lane that holds their active set and
+// the two-cell stencil reach.  Same operations, same bits.
 template <typename T, int P, int TILE>
 __global__ __launch_bounds__(kBlock) void k_euler_swept(const T* __restrict__ in,
                                                         T* __restrict__ out, int64_t n,
@@ -571,8 +571,8 @@ __global__ __launch_bounds__(kBlock) void k_euler_swept(const T* __restrict__ in
                                                         int64_t offset, EulerCoef<T> k)
 {
     constexpr int W = kWave * P, H = W / 4;
-    constexpr int NPH = P == 8 ? 4 : (P == 4 ? 2 : 1);  // region widths
-    __shared__ T img_all[NPH > 1 ? kWavesPerBlock * 6 * W : 1];
+    constexpr bool PHASED = P >= 4;
+    __shared__ T img_all[PHASED ? kWavesPerBlock * 6 * W : 1];
     NodeGeom g;
     if (!node_geom<P>(n, nnodes, node_first, offset, g)) return;
 
@@ -588,62 +588,61 @@ __global__ __launch_bounds__(kBlock) void k_euler_swept(const T* __restrict__ in
     }
     full.place(W, g.lane * P, g.base, n);
 
-    if constexpr (NPH == 1) {
+    if constexpr (!PHASED) {
         if (TILE != TILE_UP) full.down(1, H - 1, k);
         if (TILE != TILE_DOWN) {
             full.up(W, 1, H - 3, k);
             full.template stage<0>(W - 1 - 4 * (H - 1), k);
         }
     } else {
-        // region j of NPH (j < NPH) has PC = j P / NPH cells per lane and takes
-        // down pairs s <= 16 PC - 1 and up pairs m >= 16 (P - PC) + 1
+        // Regions of PC = 2, 3, .. P-1 cells per lane (widths 64 PC): region PC
+        // takes down pairs 16 PC - 15 <= s <= 16 PC - 1 (from 1 for PC = 2) and
+        // up pairs 16(P-PC)+1 <= m <= 16(P-PC)+15 (to H-3 for PC = 2), where
+        // every threshold is <= 64 PC - 1.
         T* img = img_all + (threadIdx.x >> 6) * 6 * W;
         const int lane = g.lane;
-        auto region = [&](auto pc_tag, auto&& body) {
-            constexpr int PC = decltype(pc_tag)::value;
-            EulerRegion<T, PC> r;
-            r.template load<W>(img, lane, g.base, n);
-            body(r);
-            r.template save<W>(img, lane);
-            wave_sync();
-        };
         auto reload_full = [&] {
 #pragma unroll
             for (int c = 0; c < 6; ++c)
 #pragma unroll
                 for (int p = 0; p < P; ++p) full.q[c][p] = img[c * W + lane * P + p];
         };
-        using PC1 = std::integral_constant<int, P / NPH>;
-        using PC2 = std::integral_constant<int, 2 * P / NPH>;
-        using PC3 = std::integral_constant<int, 3 * P / NPH>;
-        constexpr int S1 = 16 * PC1::value - 1, S2 = 16 * PC2::value - 1, S3 = 16 * PC3::value - 1;
-        constexpr int M1 = 16 * (P - PC1::value) + 1, M2 = 16 * (P - PC2::value) + 1,
-                      M3 = 16 * (P - PC3::value) + 1;
-        constexpr int SL = NPH == 4 ? S3 : S1;  // last region pair before the full width
-        constexpr int ML = NPH == 4 ? M3 : M1;  // first region pair after it
+        auto down_region = [&](auto pc_tag) {
+            constexpr int PC = decltype(pc_tag)::value;
+            EulerRegion<T, PC> r;
+            r.template load<W>(img, lane, g.base, n);
+            r.down(PC == 2 ? 1 : 16 * PC - 15, 16 * PC - 1, k);
+            r.template save<W>(img, lane);
+            wave_sync();
+        };
+        auto up_region = [&](auto pc_tag) {
+            constexpr int PC = decltype(pc_tag)::value;
+            EulerRegion<T, PC> r;
+            r.template load<W>(img, lane, g.base, n);
+            if constexpr (PC == 2) {
+                r.up(W, 16 * (P - 2) + 1, H - 3, k);
+                r.template stage<0>(W - 1 - 4 * (H - 1), k);
+            } else {
+                r.up(W, 16 * (P - PC) + 1, 16 * (P - PC) + 15, k);
+            }
+            r.template save<W>(img, lane);
+            wave_sync();
+        };
+        using Regions = std::make_integer_sequence<int, P - 2>;  // i -> PC = i + 2
         if (TILE != TILE_UP) {
             full.template save<W>(img, lane);
             wave_sync();
-            region(PC1{}, [&](auto& r) { r.down(1, S1, k); });
-            if constexpr (NPH == 4) {
-                region(PC2{}, [&](auto& r) { r.down(S1 + 2, S2, k); });
-                region(PC3{}, [&](auto& r) { r.down(S2 + 2, S3, k); });
-            }
+            for_each_index([&](auto i) { down_region(std::integral_constant<int, decltype(i)::value + 2>{}); },
+                           Regions{});
             reload_full();
-            full.down(SL + 2, H - 1, k);
+            full.down(16 * (P - 1) + 1, H - 1, k);
         }
         if (TILE != TILE_DOWN) {
-            full.up(W, 1, ML - 2, k);
+            full.up(W, 1, 15, k);
             full.template save<W>(img, lane);
             wave_sync();
-            if constexpr (NPH == 4) {
-                region(PC3{}, [&](auto& r) { r.up(W, M3, M2 - 2, k); });
-                region(PC2{}, [&](auto& r) { r.up(W, M2, M1 - 2, k); });
-            }
-            region(PC1{}, [&](auto& r) {
-                r.up(W, M1, H - 3, k);
-                r.template stage<0>(W - 1 - 4 * (H - 1), k);
-            });
+            for_each_index([&](auto i) { up_region(std::integral_constant<int, P - 1 - decltype(i)::value>{}); },
+                           Regions{});
             reload_full();
         }
     }

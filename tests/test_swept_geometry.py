@@ -106,10 +106,10 @@ def test_heat_region_phases_contain_active_sets(P):
             assert np.all(d[halo] > t), (pc, t)
 
 
-@pytest.mark.parametrize("P,pcs", [(4, (2,)), (8, (2, 4, 6))])
+@pytest.mark.parametrize("P,pcs", [(4, (2, 3)), (8, (2, 3, 4, 5, 6, 7))])
 def test_euler_region_phases_contain_active_sets(P, pcs):
-    """k_euler_swept: two-cell stencil like KS; regions of PC = P/2 (P = 4) or
-    P/4, P/2, 3P/4 (P = 8) cells per lane below the full width."""
+    """k_euler_swept: two-cell stencil like KS; regions of PC = 2, 3, .. P-1
+    cells per lane below the full width."""
     W, H = 64 * P, 16 * P
     S = [16 * pc - 1 for pc in pcs]
     M = [16 * (P - pc) + 1 for pc in pcs]
