@@ -61,7 +61,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=16.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
-                    help="skip the trapezoid engine and the batched workload")
+                    help="skip every extra section (same as --extras none)")
+    ap.add_argument("--extras", default="trapezoid,equations,batched,surrogate",
+                    help="comma-separated extra sections to run, or 'none'")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run one plain copy kernel over this many MiB (reads and writes "
                          "that many bytes), the known-size reference for HBM byte counters")
@@ -145,10 +147,14 @@ def main():
                      "flop_per_launch": flops,
                      "algorithmic_bytes_per_launch": 32 * n},
     }
-    if not args.no_extras:
+    extras = set() if args.no_extras else set(args.extras.split(",")) - {"none", ""}
+    if "trapezoid" in extras:
         out["trapezoid"] = _trapezoid(sw, n, dt, rng_ic, args)
+    if "equations" in extras:
         out["equations"] = _equations(sw, n)
+    if "batched" in extras:
         out["batched"] = _batched(sw)
+    if "surrogate" in extras:
         out["surrogate"] = _surrogate()
     if not args.no_cpu:
         out["cpu_baseline"] = _cpu_baseline(n, dt, rng_ic, args.cpu_seconds)
