@@ -712,22 +712,22 @@ int sw1d_synchronize(sw1d_plan* p)
 int sw1d_advance_timed(sw1d_plan* p, int64_t nsteps, double* ms)
 {
     if (!p || !ms) return fail(SW1D_EINVAL, "null argument");
-    hipEvent_t a, b;
-    HIP_OR_FAIL(hipEventCreate(&a));
-    HIP_OR_FAIL(hipEventCreate(&b));
-    HIP_OR_FAIL(hipEventRecord(a, p->stream));
-    int rc = sw1d_advance(p, nsteps);
+    hipEvent_t a = nullptr, b = nullptr;
+    hipError_t e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
+    if (e == hipSuccess) e = hipEventRecord(a, p->stream);
+    int rc = e == hipSuccess ? sw1d_advance(p, nsteps) : SW1D_EDEVICE;
     if (rc == SW1D_OK) {
-        HIP_OR_FAIL(hipEventRecord(b, p->stream));
-        HIP_OR_FAIL(hipEventSynchronize(b));
         float f = 0.f;
-        HIP_OR_FAIL(hipEventElapsedTime(&f, a, b));
-        *ms = f;
-        rc = sw1d_synchronize(p);
+        e = hipEventRecord(b, p->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(b);
+        if (e == hipSuccess) e = hipEventElapsedTime(&f, a, b);
+        if (e == hipSuccess) *ms = f;
     }
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
-    return rc;
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    if (e != hipSuccess) return fail(SW1D_EDEVICE, "timing events: %s", hipGetErrorString(e));
+    return rc == SW1D_OK ? sw1d_synchronize(p) : rc;
 }
 
 int sw1d_plan_info(const sw1d_plan* p, int64_t info[5])
@@ -978,20 +978,21 @@ int sw1d_batch_advance(sw1d_batch* B, int64_t nsteps)
 int sw1d_batch_advance_timed(sw1d_batch* B, int64_t nsteps, double* ms)
 {
     if (!B || !ms) return fail(SW1D_EINVAL, "null argument");
-    hipEvent_t e0, e1;
-    HIP_OR_FAIL(hipEventCreate(&e0));
-    HIP_OR_FAIL(hipEventCreate(&e1));
-    HIP_OR_FAIL(hipEventRecord(e0, B->stream));
-    int rc = sw1d_batch_advance(B, nsteps);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipEventRecord(e0, B->stream);
+    int rc = e == hipSuccess ? sw1d_batch_advance(B, nsteps) : SW1D_EDEVICE;
     if (rc == SW1D_OK) {
-        HIP_OR_FAIL(hipEventRecord(e1, B->stream));
-        HIP_OR_FAIL(hipEventSynchronize(e1));
         float f = 0;
-        HIP_OR_FAIL(hipEventElapsedTime(&f, e0, e1));
-        *ms = f;
+        e = hipEventRecord(e1, B->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&f, e0, e1);
+        if (e == hipSuccess) *ms = f;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (e != hipSuccess) return fail(SW1D_EDEVICE, "batch timing: %s", hipGetErrorString(e));
     return rc;
 }
 
