@@ -62,6 +62,9 @@ struct sw1d_plan {
     void* buf[2]{nullptr, nullptr};  // each nc*n REALs, component-major
     int cur = 0;                     // buffer holding the current (flat) state
     hipStream_t stream = nullptr;
+    // classic algorithm: a hipGraph of kClassicGraphSteps steps per starting buffer
+    hipGraphExec_t classic_exec[2]{nullptr, nullptr};
+    bool classic_graph_failed = false;
 
     // hybrid resources
     hipStream_t copy_stream = nullptr;
@@ -236,6 +239,44 @@ int classic_step(sw1d_plan* p)
         }
     }
     return SW1D_OK;
+}
+
+// Classic steps in blocks of kClassicGraphSteps replayed from a hipGraph (one
+// graph launch instead of 64-128 kernel launches; the kernels and their order
+// are the same, so the bits are the same).  Heat alternates its two buffers,
+// so a graph is kept per starting buffer; an even block returns to it.  With
+// per-kernel profiling on, or if capture fails, the steps are launched one by one.
+constexpr int64_t kClassicGraphSteps = 64;
+
+int classic_advance(sw1d_plan* p, int64_t nsteps)
+{
+    if (!p->profiling && !p->classic_graph_failed && nsteps >= kClassicGraphSteps) {
+        const int slot = p->cur;
+        if (!p->classic_exec[slot]) {
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal);
+            int rc = SW1D_OK;
+            for (int64_t s = 0; s < kClassicGraphSteps && e == hipSuccess && rc == SW1D_OK; ++s)
+                rc = classic_step(p);
+            const hipError_t e2 = e == hipSuccess ? hipStreamEndCapture(p->stream, &g) : e;
+            p->cur = slot;  // capture launched nothing
+            if (rc == SW1D_OK && e2 == hipSuccess && g)
+                e = hipGraphInstantiate(&p->classic_exec[slot], g, nullptr, nullptr, 0);
+            if (g) (void)hipGraphDestroy(g);
+            if (rc != SW1D_OK || e2 != hipSuccess || e != hipSuccess) {
+                p->classic_exec[slot] = nullptr;
+                p->classic_graph_failed = true;
+                (void)hipGetLastError();
+            }
+        }
+        if (p->classic_exec[slot]) {
+            for (; nsteps >= kClassicGraphSteps; nsteps -= kClassicGraphSteps)
+                HIP_OR_FAIL(hipGraphLaunch(p->classic_exec[slot], p->stream));
+        }
+    }
+    int rc = SW1D_OK;
+    for (int64_t s = 0; s < nsteps && rc == SW1D_OK; ++s) rc = classic_step(p);
+    return rc;
 }
 
 int swept_launch(sw1d_plan* p, int tile, bool shifted, int64_t node_first)
@@ -464,6 +505,8 @@ void release(sw1d_plan* p)
     if (p->ev_in) (void)hipEventDestroy(p->ev_in);
     if (p->ev_out) (void)hipEventDestroy(p->ev_out);
     if (p->copy_stream) (void)hipStreamDestroy(p->copy_stream);
+    for (auto x : p->classic_exec)
+        if (x) (void)hipGraphExecDestroy(x);
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
 }
@@ -690,8 +733,7 @@ int sw1d_advance(sw1d_plan* p, int64_t nsteps)
     if (nsteps < 0) return fail(SW1D_EINVAL, "nsteps must be >= 0");
     int rc;
     if (p->alg == SW1D_CLASSIC) {
-        rc = SW1D_OK;
-        for (int64_t s = 0; s < nsteps && rc == SW1D_OK; ++s) rc = classic_step(p);
+        rc = classic_advance(p, nsteps);
     } else if (p->alg == SW1D_TRAPEZOID) {
         rc = trapezoid_advance(p, nsteps);
     } else {

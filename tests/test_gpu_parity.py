@@ -146,3 +146,23 @@ def test_trapezoid_parity(sw, orc, eq, bc, prec, W, depth):
     want = _oracle(orc, eq, u0, nsteps, dt, dtype, bc, n)
     bad = np.flatnonzero(got != want)
     assert bad.size == 0, f"{bad.size} mismatches, first at {bad[:8]}"
+
+
+@pytest.mark.parametrize("eq", [0, 1, 2])
+def test_classic_graph_blocks(sw, orc, eq):
+    """The classic algorithm replays blocks of 64 steps from a hipGraph (one
+    per starting buffer for heat, which alternates buffers); advancing in
+    pieces that start on either buffer gives the oracle's bits."""
+    n = 4096 if eq != 2 else 2048
+    dtype = np.float64
+    dt = _dt(eq, n, 0)
+    u0 = _ic(orc, eq, n, dtype, 0)
+    pieces = [3, 130, 64, 1, 200]
+    _say(f"classic graph blocks eq={eq} n={n} pieces={pieces}")
+    with sw.Plan(eq, n, dt, algorithm=0) as p:
+        p.set_state(u0)
+        for k in pieces:
+            p.advance(k)
+        got = p.get_state()
+    want = _oracle(orc, eq, u0, sum(pieces), dt, dtype, 0, n)
+    assert np.array_equal(got, want)
