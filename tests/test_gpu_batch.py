@@ -18,7 +18,8 @@ def wl():
 def test_batch_matches_oracle(sw, orc, wl, prec, nsteps):
     count = 40
     eq, nx, dt, dx = wl.batch_problems(count, seed=prec + 5, lo=7.0, hi=13.0)
-    tps = np.array([(128, 256, 512)[i % 3] for i in range(count)], np.int32)
+    # every tile width, also wider than the problem (windows wrap several times)
+    tps = np.array([(128, 256, 512, 1024)[i % 4] for i in range(count)], np.int32)
     dtype = np.float64 if prec == 0 else np.float32
     ics = [u.astype(dtype) for u in wl.batch_ics(eq, nx, dx)]
     print(f"batch prec={prec} steps={nsteps} sizes={nx.min()}..{nx.max()}", flush=True)
