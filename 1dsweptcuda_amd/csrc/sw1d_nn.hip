@@ -190,11 +190,15 @@ __global__ __launch_bounds__(NT) void k_mid_fwd(const __bf16* __restrict__ Ain, 
     }
 }
 
-// Same product over 256 positions per workgroup: the 40 KB of staged weights
-// serve four times the positions, each wave owns 64 positions x 64 channels
-// (16 accumulator tiles) and reuses every A and B fragment four times.
+// Same product over 256 positions per workgroup of 8 waves: the 40 KB of
+// staged weights serve four times the positions of k_mid_fwd, each wave owns
+// 32 positions x 64 channels (8 accumulator tiles, so four waves fit per SIMD
+// and one workgroup's MFMAs overlap another's GELU epilogue).
 constexpr int TILE_F = 256;
-static_assert(NT == TILE_F, "the output-layer epilogue maps one thread to one position");
+constexpr int NTW = 512;            // threads of the wide kernels
+constexpr int WROWS = TILE_F / (NTW / 64);  // positions per wave
+constexpr int MB = WROWS / 16;      // 16-row MFMA blocks per wave
+static_assert(NTW >= TILE_F, "the output-layer epilogue maps one thread to one position");
 // LDS rows of 64 BF16 (8 chunks of 16 bytes) with chunk c of row r stored at
 // chunk c ^ (r & 7): the 16 rows read together by a fragment load hit 8
 // different chunk slots, so no padding is needed (2 workgroups fit per CU).
@@ -203,7 +207,7 @@ __device__ __forceinline__ int swz(int r, int chunk) { return r * C + ((chunk ^ 
 // rows x0-2 .. x0+TILE_F+1 of one sample's [X][64] BF16 activation -> sA
 __device__ __forceinline__ void stage_act(__bf16* sA, const __bf16* __restrict__ Ab, int x0, int X)
 {
-    for (int q = threadIdx.x; q < (TILE_F + 4) * (C / 8); q += NT) {
+    for (int q = threadIdx.x; q < (TILE_F + 4) * (C / 8); q += NTW) {
         const int r = q / (C / 8), c8 = q % (C / 8);
         const int x = wrapx(x0 - 2 + r, X);
         *reinterpret_cast<uint4*>(&sA[swz(r, c8)]) =
@@ -213,7 +217,7 @@ __device__ __forceinline__ void stage_act(__bf16* sA, const __bf16* __restrict__
 // a layer's packed weights [t][co][ci] -> sW (rows t*64 + co)
 __device__ __forceinline__ void stage_w(__bf16* sW, const __bf16* __restrict__ Wf)
 {
-    for (int q = threadIdx.x; q < TAPS * C * (C / 8); q += NT) {
+    for (int q = threadIdx.x; q < TAPS * C * (C / 8); q += NTW) {
         const int r = q / (C / 8), c8 = q % (C / 8);
         *reinterpret_cast<uint4*>(&sW[swz(r, c8)]) = *reinterpret_cast<const uint4*>(&Wf[r * C + c8 * 8]);
     }
@@ -226,8 +230,8 @@ __device__ __forceinline__ void stage_l1(__bf16* sA, float* sIn, const TIN* __re
                                          const float* __restrict__ w1, const float* __restrict__ b1, int x0,
                                          int X)
 {
-    for (int q = threadIdx.x; q < TILE_F + 8; q += NT) sIn[q] = (float)ub[wrapx(x0 - 4 + q, X)] * inv_scale;
-    const int c8 = threadIdx.x % (C / 8);  // fixed per thread: NT is a multiple of 8
+    for (int q = threadIdx.x; q < TILE_F + 8; q += NTW) sIn[q] = (float)ub[wrapx(x0 - 4 + q, X)] * inv_scale;
+    const int c8 = threadIdx.x % (C / 8);  // fixed per thread: NTW is a multiple of 8
     float w[8][TAPS], bb[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
@@ -236,7 +240,7 @@ __device__ __forceinline__ void stage_l1(__bf16* sA, float* sIn, const TIN* __re
         for (int t = 0; t < TAPS; ++t) w[k][t] = w1[(c8 * 8 + k) * TAPS + t];
     }
     __syncthreads();
-    for (int r = threadIdx.x / (C / 8); r < TILE_F + 4; r += NT / (C / 8)) {
+    for (int r = threadIdx.x / (C / 8); r < TILE_F + 4; r += NTW / (C / 8)) {
         bf16x8 v;
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
@@ -248,27 +252,27 @@ __device__ __forceinline__ void stage_l1(__bf16* sA, float* sIn, const TIN* __re
         *reinterpret_cast<bf16x8*>(&sA[swz(r, c8)]) = v;
     }
 }
-// acc[mb][nb] = the wave's 64 positions x 64 channels of sum_t A(x+t-2) W[t]^T
-__device__ __forceinline__ void mfma_tile(const __bf16* sA, const __bf16* sW, f32x4 (&acc)[4][4])
+// acc[mb][nb] = the wave's WROWS positions x 64 channels of sum_t A(x+t-2) W[t]^T
+__device__ __forceinline__ void mfma_tile(const __bf16* sA, const __bf16* sW, f32x4 (&acc)[MB][4])
 {
     const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int m0 = 64 * w, li = l % 16, lq = l / 16;
+    const int m0 = WROWS * w, li = l % 16, lq = l / 16;
 #pragma unroll
-    for (int mb = 0; mb < 4; ++mb)
+    for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
         for (int nb = 0; nb < 4; ++nb) acc[mb][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll 2
     for (int s = 0; s < TAPS * 2; ++s) {
         const int t = s >> 1, chunk = (s & 1) * 4 + lq;  // K = channels (s&1)*32 + 8*lq ..
-        bf16x8 a[4], bv[4];
+        bf16x8 a[MB], bv[4];
 #pragma unroll
-        for (int mb = 0; mb < 4; ++mb)
+        for (int mb = 0; mb < MB; ++mb)
             a[mb] = *reinterpret_cast<const bf16x8*>(&sA[swz(m0 + 16 * mb + li + t, chunk)]);
 #pragma unroll
         for (int nb = 0; nb < 4; ++nb)
             bv[nb] = *reinterpret_cast<const bf16x8*>(&sW[swz(t * C + 16 * nb + li, chunk)]);
 #pragma unroll
-        for (int mb = 0; mb < 4; ++mb)
+        for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
             for (int nb = 0; nb < 4; ++nb)
                 acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mb], bv[nb], acc[mb][nb], 0, 0, 0);
@@ -278,17 +282,17 @@ __device__ __forceinline__ void mfma_tile(const __bf16* sA, const __bf16* sW, f3
 // position in the tile, swizzled); FP32 pre-activations to Z (rows zrow0 + r)
 // when STORE_Z.  sO may alias sA once every wave has finished mfma_tile.
 template <bool STORE_Z>
-__device__ __forceinline__ void epilogue(const f32x4 (&acc)[4][4], const float* __restrict__ bias, __bf16* sO,
+__device__ __forceinline__ void epilogue(const f32x4 (&acc)[MB][4], const float* __restrict__ bias, __bf16* sO,
                                          float* __restrict__ Z, int64_t zrow0)
 {
     const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int m0 = 64 * w, li = l % 16, lq = l / 16;
+    const int m0 = WROWS * w, li = l % 16, lq = l / 16;
 #pragma unroll
     for (int nb = 0; nb < 4; ++nb) {
         const int co = 16 * nb + li;
         const float bc = bias[co];
 #pragma unroll
-        for (int mb = 0; mb < 4; ++mb)
+        for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int r = m0 + 16 * mb + 4 * lq + j;
@@ -301,7 +305,7 @@ __device__ __forceinline__ void epilogue(const f32x4 (&acc)[4][4], const float* 
 // the LDS tile -> rows x0 .. x0+TILE_F-1 of a [X][64] activation (16-byte stores)
 __device__ __forceinline__ void store_act(const __bf16* sO, __bf16* __restrict__ Ab, int x0)
 {
-    for (int q = threadIdx.x; q < TILE_F * (C / 8); q += NT) {
+    for (int q = threadIdx.x; q < TILE_F * (C / 8); q += NTW) {
         const int r = q / (C / 8), c8 = q % (C / 8);
         *reinterpret_cast<uint4*>(&Ab[(int64_t)(x0 + r) * C + c8 * 8]) =
             *reinterpret_cast<const uint4*>(&sO[swz(r, c8)]);
@@ -311,7 +315,7 @@ __device__ __forceinline__ void store_act(const __bf16* sO, __bf16* __restrict__
 // One 64 -> 64 layer over 256 positions.  STORE_Z keeps the pre-activations
 // for the backward pass (training only).
 template <bool STORE_Z>
-__global__ __launch_bounds__(NT) void k_mid_fwd_wide(const __bf16* __restrict__ Ain,
+__global__ __launch_bounds__(NTW) void k_mid_fwd_wide(const __bf16* __restrict__ Ain,
                                                      const __bf16* __restrict__ Wf,
                                                      const float* __restrict__ bias, float* __restrict__ Z,
                                                      __bf16* __restrict__ Aout, int X)
@@ -322,7 +326,7 @@ __global__ __launch_bounds__(NT) void k_mid_fwd_wide(const __bf16* __restrict__ 
     stage_act(sA, Ain + (int64_t)b * X * C, x0, X);
     stage_w(sW, Wf);
     __syncthreads();
-    f32x4 acc[4][4];
+    f32x4 acc[MB][4];
     mfma_tile(sA, sW, acc);
     __syncthreads();
     epilogue<STORE_Z>(acc, bias, sA, Z, (int64_t)b * X + x0);
@@ -333,7 +337,7 @@ __global__ __launch_bounds__(NT) void k_mid_fwd_wide(const __bf16* __restrict__ 
 // Layers 1 and 2 in one pass (inference): layer 1 is evaluated into the LDS
 // tile from the network input, so its activation never goes to memory.
 template <typename TIN>
-__global__ __launch_bounds__(NT) void k_l12_fwd(const TIN* __restrict__ in, float inv_scale,
+__global__ __launch_bounds__(NTW) void k_l12_fwd(const TIN* __restrict__ in, float inv_scale,
                                                 const float* __restrict__ w1, const float* __restrict__ b1,
                                                 const __bf16* __restrict__ Wf, const float* __restrict__ bias,
                                                 __bf16* __restrict__ Aout, int X)
@@ -345,7 +349,7 @@ __global__ __launch_bounds__(NT) void k_l12_fwd(const TIN* __restrict__ in, floa
     stage_w(sW, Wf);
     stage_l1<TIN>(sA, sIn, in + (int64_t)b * X, inv_scale, w1, b1, x0, X);
     __syncthreads();
-    f32x4 acc[4][4];
+    f32x4 acc[MB][4];
     mfma_tile(sA, sW, acc);
     __syncthreads();
     epilogue<false>(acc, bias, sA, nullptr, 0);
@@ -357,7 +361,7 @@ __global__ __launch_bounds__(NT) void k_l12_fwd(const TIN* __restrict__ in, floa
 // LDS and each thread forms, for its position x, the five tap sums
 //   S[t][b*X + x] = sum_c w6[t][c] A5[x][c],
 // from which du[x] = b6 + sum_t S[t][x+t-2] (k_l6_out).
-__global__ __launch_bounds__(NT) void k_l56_fwd(const __bf16* __restrict__ Ain, const __bf16* __restrict__ Wf,
+__global__ __launch_bounds__(NTW) void k_l56_fwd(const __bf16* __restrict__ Ain, const __bf16* __restrict__ Wf,
                                                 const float* __restrict__ bias, const float* __restrict__ w6,
                                                 float* __restrict__ S, int X, int64_t plane)
 {
@@ -367,14 +371,15 @@ __global__ __launch_bounds__(NT) void k_l56_fwd(const __bf16* __restrict__ Ain, 
     const int b = blockIdx.y, x0 = blockIdx.x * TILE_F;
     stage_act(sA, Ain + (int64_t)b * X * C, x0, X);
     stage_w(sW, Wf);
-    for (int q = threadIdx.x; q < TAPS * C; q += NT) sW6[q] = w6[q];
+    for (int q = threadIdx.x; q < TAPS * C; q += NTW) sW6[q] = w6[q];
     __syncthreads();
-    f32x4 acc[4][4];
+    f32x4 acc[MB][4];
     mfma_tile(sA, sW, acc);
     __syncthreads();
     epilogue<false>(acc, bias, sA, nullptr, 0);
     __syncthreads();
     const int r = threadIdx.x;
+    if (r >= TILE_F) return;
     float sum[TAPS] = {0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int c8 = 0; c8 < C / 8; ++c8) {
@@ -803,10 +808,10 @@ int nn_forward(sw1d_nn* N, int B, bool store_z = true)
         const float* bias = N->p + P_MID + l * MID_SZ + TAPS * C * C;
         if (X % TILE_F == 0) {
             if (store_z)
-                hipLaunchKernelGGL(k_mid_fwd_wide<true>, dim3(X / TILE_F, B), dim3(NT), 0, N->stream, N->A[l],
+                hipLaunchKernelGGL(k_mid_fwd_wide<true>, dim3(X / TILE_F, B), dim3(NTW), 0, N->stream, N->A[l],
                                    N->wf[l], bias, N->Z[l + 1], N->A[l + 1], X);
             else
-                hipLaunchKernelGGL(k_mid_fwd_wide<false>, dim3(X / TILE_F, B), dim3(NT), 0, N->stream, N->A[l],
+                hipLaunchKernelGGL(k_mid_fwd_wide<false>, dim3(X / TILE_F, B), dim3(NTW), 0, N->stream, N->A[l],
                                    N->wf[l], bias, N->Z[l + 1], N->A[l + 1], X);
         } else {
             hipLaunchKernelGGL(k_mid_fwd, dim3(X / TILE, B), dim3(NT), 0, N->stream, N->A[l], N->wf[l], bias,
@@ -843,15 +848,15 @@ int nn_infer(sw1d_nn* N, int B, double* u)
     const float* P = N->p;
     auto bias = [&](int l) { return P + P_MID + l * MID_SZ + TAPS * C * C; };
     if (u)
-        hipLaunchKernelGGL(k_l12_fwd<double>, grid, dim3(NT), 0, N->stream, u, (float)(1.0 / N->in_scale),
+        hipLaunchKernelGGL(k_l12_fwd<double>, grid, dim3(NTW), 0, N->stream, u, (float)(1.0 / N->in_scale),
                            P + P_W1, P + P_B1, N->wf[0], bias(0), N->A[1], X);
     else
-        hipLaunchKernelGGL(k_l12_fwd<float>, grid, dim3(NT), 0, N->stream, N->x, 1.0f, P + P_W1, P + P_B1,
+        hipLaunchKernelGGL(k_l12_fwd<float>, grid, dim3(NTW), 0, N->stream, N->x, 1.0f, P + P_W1, P + P_B1,
                            N->wf[0], bias(0), N->A[1], X);
     for (int l = 1; l <= 2; ++l)
-        hipLaunchKernelGGL(k_mid_fwd_wide<false>, grid, dim3(NT), 0, N->stream, N->A[l], N->wf[l], bias(l),
+        hipLaunchKernelGGL(k_mid_fwd_wide<false>, grid, dim3(NTW), 0, N->stream, N->A[l], N->wf[l], bias(l),
                            nullptr, N->A[l + 1], X);
-    hipLaunchKernelGGL(k_l56_fwd, grid, dim3(NT), 0, N->stream, N->A[3], N->wf[3], bias(3), P + P_W6, N->taps,
+    hipLaunchKernelGGL(k_l56_fwd, grid, dim3(NTW), 0, N->stream, N->A[3], N->wf[3], bias(3), P + P_W6, N->taps,
                        X, plane);
     if (u)
         hipLaunchKernelGGL(k_l6_out<true>, dim3(nblk(plane)), dim3(NT), 0, N->stream, N->taps, P + P_B6,
