@@ -76,11 +76,25 @@ __device__ inline float erf_as(float x)
     return copysignf(fmaf(-q * t, e, 1.0f), x);
 }
 
-// GELU(z) = z/2 (1 + erf(z/sqrt 2)) and its derivative Phi(z) + z phi(z).
+// erf(|z| / sqrt 2), the same approximation with the 1/sqrt 2 folded into
+// its constants (p / sqrt 2 and -log2(e) / 2).
+__device__ inline float erf_abs_scaled(float z)
+{
+    const float t = __builtin_amdgcn_rcpf(fmaf(0.23164189f, fabsf(z), 1.0f));
+    float q = fmaf(1.061405429f, t, -1.453152027f);
+    q = fmaf(q, t, 1.421413741f);
+    q = fmaf(q, t, -0.284496736f);
+    q = fmaf(q, t, 0.254829592f);
+    const float e = __builtin_amdgcn_exp2f((-0.72134752044448170f * z) * z);  // exp(-z^2/2)
+    return fmaf(-q * t, e, 1.0f);
+}
+
+// GELU(z) = z/2 (1 + erf(z/sqrt 2)) = h + |h| erf(|z|/sqrt 2), h = z/2 (h and
+// z share their sign), and its derivative Phi(z) + z phi(z).
 __device__ inline float gelu(float z)
 {
     const float h = 0.5f * z;
-    return fmaf(h, erf_as(z * 0.70710678118654752f), h);
+    return fmaf(fabsf(h), erf_abs_scaled(z), h);
 }
 __device__ inline float gelu_d(float z)
 {
