@@ -997,8 +997,8 @@ int sw1d_nn_create(sw1d_nn** out, int32_t nx, int32_t max_batch, uint64_t seed)
 {
     if (!out) return nn_fail(SW1D_EINVAL, "null argument");
     *out = nullptr;
-    if (nx < TILE || nx % TILE != 0 || max_batch < 1 || max_batch > 65535)
-        return nn_fail(SW1D_EINVAL, "nx must be a positive multiple of 64 and 1 <= max_batch <= 65535");
+    if (nx < TILE || nx % TILE != 0 || nx > (1 << 24) || max_batch < 1 || max_batch > 65535)
+        return nn_fail(SW1D_EINVAL, "nx must be a multiple of 64 in [64, 2^24] and 1 <= max_batch <= 65535");
     if (g_mfma_ok == 0 && sw1d_nn_device_check() < 0) return SW1D_EDEVICE;
     if (g_mfma_ok != 1) return nn_fail(SW1D_EDEVICE, "MFMA operand layout check failed on this device");
     sw1d_nn* N = new sw1d_nn();

@@ -173,7 +173,7 @@ const char* sw1d_nn_last_error(void);
 int sw1d_nn_param_count(void);
 /* Checks the MFMA operand layout the kernels assume.  1 = pass. */
 int sw1d_nn_device_check(void);
-/* nx a positive multiple of 64 (multiples of 256 use the fused inference
+/* nx a multiple of 64 in [64, 2^24] (multiples of 256 use the fused inference
  * kernels), 1 <= max_batch <= 65535 samples per forward / training step. */
 int sw1d_nn_create(sw1d_nn** nn, int32_t nx, int32_t max_batch, uint64_t seed);
 int sw1d_nn_destroy(sw1d_nn* nn);
