@@ -361,12 +361,12 @@ struct KsRegion {
 
 // Swept KS tiles: one wave per node of W = 64*P points, lane l owning points
 // l*P .. l*P+P-1 in VGPRs.  A whole diamond runs W/2 stage levels but its
-// active set is W wide only in the middle; for P >= 8 the node's image is
+// active set is W wide only in the middle; for P >= 4 the node's image is
 // kept in LDS and the levels run on centred regions of 2, 3, .. P points per
 // lane, each level on the narrowest region that holds its active set and the
 // two-point stencil reach.  Same operations on the same values as the
 // one-width form, so the same bits, with 46% fewer point evaluations at
-// P = 16 (42% at P = 8).
+// P = 16 (42% at P = 8, 31% at P = 4).
 template <typename T, int P, int TILE>
 __global__ __launch_bounds__(kBlock) void k_ks_swept(const T* __restrict__ in,
                                                      T* __restrict__ out, int64_t n,
@@ -374,7 +374,7 @@ __global__ __launch_bounds__(kBlock) void k_ks_swept(const T* __restrict__ in,
                                                      int64_t offset, KsCoef<T> k)
 {
     constexpr int W = kWave * P, H = W / 4;
-    constexpr bool PHASED = P >= 8 && P % 4 == 0;
+    constexpr bool PHASED = P >= 4;
     __shared__ T img_all[PHASED ? kWavesPerBlock * 2 * W : 1];
     NodeGeom g;
     if (!node_geom<P>(n, nnodes, node_first, offset, g)) return;

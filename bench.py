@@ -62,7 +62,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip every extra section (same as --extras none)")
-    ap.add_argument("--extras", default="trapezoid,equations,batched,surrogate",
+    ap.add_argument("--extras", default="trapezoid,equations,sizes,batched,surrogate",
                     help="comma-separated extra sections to run, or 'none'")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run one plain copy kernel over this many MiB (reads and writes "
@@ -152,6 +152,8 @@ def main():
         out["trapezoid"] = _trapezoid(sw, n, dt, rng_ic, args)
     if "equations" in extras:
         out["equations"] = _equations(sw, n)
+    if "sizes" in extras:
+        out["sizes"] = _sizes(sw, args.seed)
     if "batched" in extras:
         out["batched"] = _batched(sw)
     if "surrogate" in extras:
@@ -201,6 +203,35 @@ def _equations(sw, n):
             row[{sw.CLASSIC: "classic", sw.SWEPT: "swept", sw.TRAPEZOID: "trapezoid"}[alg]] = n * k / secs
         row["swept_over_classic"] = row["swept"] / row["classic"]
         res[name] = row
+    res["unit"] = "point-updates/s"
+    return res
+
+
+def _sizes(sw, seed, steps=20000):
+    """The KS grid of the reference's performance sweeps, FP64: nX = 2^14 ..
+    2^20 with swept node widths 128 .. 1024 (the reference's tpb), the
+    classic algorithm and the register variant (alg 2), 20000 steps each."""
+    res = {}
+    for lg in (14, 16, 18, 20):
+        n = 1 << lg
+        u0 = _ks_ic(n, seed)
+        row = {"swept": {}}
+        for alg, name, widths in ((sw.CLASSIC, "classic", (256,)), (sw.ALT, "register", (128,)),
+                                  (sw.SWEPT, "swept", (128, 256, 512, 1024))):
+            for W in widths:
+                p = sw.Plan(sw.KS, n, 0.005, algorithm=alg, tpb=W, node_width=W if alg == sw.SWEPT else 0)
+                p.set_state(u0)
+                p.advance(steps // 10)
+                secs = _timed(p, steps)
+                p.close()
+                if alg == sw.SWEPT:
+                    row["swept"][str(W)] = n * steps / secs
+                else:
+                    row[name] = n * steps / secs
+        best = max(row["swept"], key=row["swept"].get)
+        row["best_swept"] = {"node_width": int(best), "value": row["swept"][best]}
+        row["best_swept_over_classic"] = row["swept"][best] / row["classic"]
+        res["2^%d" % lg] = row
     res["unit"] = "point-updates/s"
     return res
 

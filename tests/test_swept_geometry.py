@@ -66,9 +66,9 @@ def test_emulator_detects_a_wrong_active_set(orc, monkeypatch):
     assert not np.array_equal(bad, good)
 
 
-@pytest.mark.parametrize("P", [8, 16])
+@pytest.mark.parametrize("P", [4, 8, 16])
 def test_ks_region_phases_contain_active_sets(P):
-    """The KS diamond kernel (k_ks_swept, P >= 8) runs its stage levels on
+    """The KS diamond kernel (k_ks_swept, P >= 4) runs its stage levels on
     centred regions of 2, 3, .. P-1 points per lane below the full width; the
     same thresholds as Euler's two-cell stencil."""
     test_euler_region_phases_contain_active_sets(P, tuple(range(2, P)))
