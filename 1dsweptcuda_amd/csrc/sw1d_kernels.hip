@@ -189,9 +189,9 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Swept heat tiles: one wave per node of W = 64*P points.  For P >= 4 the
+// Swept heat tiles: one wave per node of W = 64*P points.  For P >= 2 the
 // node's image sits in LDS and the diamond's levels run on centred regions of
-// 1, 2, 3 (P = 4) or 2, 4, .. P (P >= 8) points per lane, each level on the
+// 1 .. P (P <= 4) or 2, 4, .. P (P >= 8) points per lane, each level on the
 // narrowest region holding its active set and the one-point stencil reach:
 // the same operations, so the same bits, on fewer points.
 template <typename T, int P, int TILE, bool MIRROR>
@@ -201,7 +201,7 @@ __global__ __launch_bounds__(kBlock) void k_heat_swept(const T* __restrict__ in,
                                                        int64_t offset, T fo, T fo2)
 {
     constexpr int W = kWave * P, H = W / 2;
-    constexpr bool PHASED = P >= 4 && P % 4 == 0;
+    constexpr bool PHASED = P >= 2;
     __shared__ T img_all[PHASED ? kWavesPerBlock * 2 * W : 1];
     NodeGeom g;
     if (!node_geom<P>(n, nnodes, node_first, offset, g)) return;

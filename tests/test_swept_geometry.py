@@ -74,11 +74,11 @@ def test_ks_region_phases_contain_active_sets(P):
     test_euler_region_phases_contain_active_sets(P, tuple(range(2, P)))
 
 
-@pytest.mark.parametrize("P", [4, 8, 16])
+@pytest.mark.parametrize("P", [2, 4, 8, 16])
 def test_heat_region_phases_contain_active_sets(P):
-    """Same check for the heat diamond kernel (k_heat_swept, P >= 4): one-point
+    """Same check for the heat diamond kernel (k_heat_swept, P >= 2): one-point
     stencil, thresholds 2s-1, 2s+1 down and W-1-2m, W-3-2m up; regions of
-    1, 2, 3 (P = 4) or 2, 4, .. P-2 (P >= 8) points per lane."""
+    1 .. P-1 (P <= 4) or 2, 4, .. P-2 (P >= 8) points per lane."""
     W, H = 64 * P, 32 * P
     step = 2 if P >= 8 else 1
     pcs = list(range(step, P, step))
