@@ -210,7 +210,8 @@ def _equations(sw, n):
 def _sizes(sw, seed, steps=20000):
     """The KS grid of the reference's performance sweeps, FP64: nX = 2^14 ..
     2^20 with swept node widths 128 .. 1024 (the reference's tpb), the
-    classic algorithm and the register variant (alg 2), 20000 steps each."""
+    classic algorithm and the register variant (alg 2), 20000 steps each;
+    and FP32 at 2^20."""
     res = {}
     for lg in (14, 16, 18, 20):
         n = 1 << lg
@@ -232,6 +233,18 @@ def _sizes(sw, seed, steps=20000):
         row["best_swept"] = {"node_width": int(best), "value": row["swept"][best]}
         row["best_swept_over_classic"] = row["swept"][best] / row["classic"]
         res["2^%d" % lg] = row
+    # FP32 at the largest size: classic and the swept node widths that can win there
+    n = 1 << 20
+    u0 = _ks_ic(n, seed).astype(np.float32)
+    row = {}
+    for alg, W, key in ((sw.CLASSIC, 256, "classic"), (sw.SWEPT, 512, "swept_512"), (sw.SWEPT, 1024, "swept_1024")):
+        p = sw.Plan(sw.KS, n, 0.005, precision=sw.F32, algorithm=alg, tpb=W, node_width=W if alg == sw.SWEPT else 0)
+        p.set_state(u0)
+        p.advance(steps // 10)
+        secs = _timed(p, steps)
+        p.close()
+        row[key] = n * steps / secs
+    res["f32_2^20"] = row
     res["unit"] = "point-updates/s"
     return res
 
