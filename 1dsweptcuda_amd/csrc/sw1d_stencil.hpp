@@ -3,7 +3,7 @@
 // values as arguments (no array indexing) so that the kernels can feed them
 // from registers.  Operand order follows the reference expressions exactly and
 // all code is compiled with -ffp-contract=off: each operator is one IEEE
-// operation (the KS stencil's explicit fused multiply-adds one each), on the
+// operation (the heat and KS stencils' explicit fused multiply-adds one each), on the
 // GPU and on the host alike, so every path of this library produces the same
 // bits.
 #pragma once
@@ -15,12 +15,17 @@
 
 namespace sw1d {
 
+// fused multiply-add, one rounding (FMA instructions on the GPU and the host)
+SW1D_HD double fmadd(double a, double b, double c) { return __builtin_fma(a, b, c); }
+SW1D_HD float fmadd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
 // ------------------------------------------------------------------ heat ---
-// Heat1D_SweptShared.cu:121-130 (execFunc): Fo*(T[l]+T[r]) + (1-2Fo)*T[c].
+// Heat1D_SweptShared.cu:121-130 (execFunc): Fo*(T[l]+T[r]) + (1-2Fo)*T[c],
+// as fma(Fo, T[l]+T[r], (1-2Fo)*T[c]) (the CPU oracle uses the same form).
 template <typename T>
 SW1D_HD T heat_ftcs(T l, T c, T r, T fo, T fo2)
 {
-    return fo * (l + r) + fo2 * c;
+    return fmadd(fo, l + r, fo2 * c);
 }
 
 // -------------------------------------------------------------------- KS ---
@@ -30,9 +35,6 @@ SW1D_HD T heat_ftcs(T l, T c, T r, T fo, T fo2)
 // evaluated with fused multiply-adds at fixed places (one rounding each; the
 // CPU oracle uses the same ones): twelve FP operations per stage.
 template <typename T> struct KsCoef { T dx4x, dx2, dx4, dt, dth; };
-
-SW1D_HD double fmadd(double a, double b, double c) { return __builtin_fma(a, b, c); }
-SW1D_HD float fmadd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
 template <typename T>
 SW1D_HD T ks_rhs(T ll, T l, T c, T r, T rr, const KsCoef<T>& k)

@@ -32,9 +32,9 @@
 #define M_PI 3.14159265358979323846
 #endif
 
-/* The KS stage is built twice, for CPUs with the FMA instructions and for
- * those without (glibc's correctly rounded fma(), the same results, slower);
- * the loader picks the clone once. */
+/* The heat and KS stages are built twice, for CPUs with the FMA instructions
+ * and for those without (glibc's correctly rounded fma(), the same results,
+ * slower); the loader picks the clone once. */
 #if defined(__x86_64__)
 #define ORC_FMA_CLONES __attribute__((target_clones("fma", "default")))
 #else
