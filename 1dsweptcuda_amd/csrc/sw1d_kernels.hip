@@ -104,11 +104,11 @@ struct HeatRegion {
     int d[PC];
     bool g0, gN;  // this lane's first / last point is global point 0 / n-1
 
-    __device__ __forceinline__ void stage(int thr, const T (&src)[PC], const T (&hl)[1], const T (&hr)[1],
+    __device__ __forceinline__ void stage(int thr, const T (&src)[PC], T (&hl)[1], T (&hr)[1],
                                           T (&dst)[PC], T fo, T fo2)
     {
         T e[PC + 2];
-        gather<T, PC, 1>(src, hl, hr, e);
+        gather_carry<T, PC, 1>(src, hl, hr, e);
         // Insulated ends (Heat:140-142): the left neighbour of point 0 is
         // point 1, the right neighbour of point n-1 is point n-2.
         if (MIRROR) {
@@ -282,8 +282,8 @@ __global__ __launch_bounds__(kBlock) void k_heat_swept(const T* __restrict__ in,
 
 // ------------------------------------------------------------- KS tiles ---
 // One RK2 stage pair on PC points per lane.  e[] holds the lane's points with
-// two neighbours on each side (DPP from the adjacent lanes, hl / hr at the
-// ends of the region); d[p] is the point's distance code |2x - (W-1)| from
+// two neighbours on each side (DPP from the adjacent lanes; the region's
+// halos in lane 0 / lane 63 of the carried neighbour registers); d[p] is the point's distance code |2x - (W-1)| from
 // the node centre, and a stage with threshold thr updates the points with
 // d <= thr (the diamond's active set) and keeps the others.
 template <typename T, int PC>
@@ -296,7 +296,7 @@ struct KsRegion {
     __device__ __forceinline__ void stage_a(int thr, const KsCoef<T>& k)
     {
         T e[PC + 4];
-        gather<T, PC, 2>(u, hlu, hru, e);
+        gather_carry<T, PC, 2>(u, hlu, hru, e);
 #pragma unroll
         for (int p = 0; p < PC; ++p) {
             const T nv = ks_predict(e[p], e[p + 1], e[p + 2], e[p + 3], e[p + 4], k);
@@ -307,7 +307,7 @@ struct KsRegion {
     __device__ __forceinline__ void stage_b(int thr, const KsCoef<T>& k)
     {
         T e[PC + 4];
-        gather<T, PC, 2>(uh, hlh, hrh, e);
+        gather_carry<T, PC, 2>(uh, hlh, hrh, e);
 #pragma unroll
         for (int p = 0; p < PC; ++p) {
             const T nv = ks_correct(u[p], e[p], e[p + 1], e[p + 2], e[p + 3], e[p + 4], k);
@@ -475,9 +475,9 @@ struct EulerRegion {
     {
         constexpr int dst = 3 - src;
         T ex[PC + 4], ey[PC + 4], ez[PC + 4];
-        gather<T, PC, 2>(q[src + 0], hl[src + 0], hr[src + 0], ex);
-        gather<T, PC, 2>(q[src + 1], hl[src + 1], hr[src + 1], ey);
-        gather<T, PC, 2>(q[src + 2], hl[src + 2], hr[src + 2], ez);
+        gather_carry<T, PC, 2>(q[src + 0], hl[src + 0], hr[src + 0], ex);
+        gather_carry<T, PC, 2>(q[src + 1], hl[src + 1], hr[src + 1], ey);
+        gather_carry<T, PC, 2>(q[src + 2], hl[src + 2], hr[src + 2], ez);
         auto cell = [&](int j) { return Q3<T>{ex[j], ey[j], ez[j]}; };
         // pressure differences between ext cells j and j+1
         T pd[PC + 3];

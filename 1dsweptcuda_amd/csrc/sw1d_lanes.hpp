@@ -55,5 +55,26 @@ __device__ __forceinline__ void gather(const T (&v)[P], const T (&hl)[R], const 
     }
 }
 
+// gather with the neighbour registers carried from one call to the next:
+// nl / nr hold the halo in lane 0 / lane 63, which the shift never writes,
+// and the last shifted values in the other lanes.  The shift's destination
+// is then its own `old` operand, so no copy of the halo precedes it; the
+// values in e[] are those of gather() with hl = nl, hr = nr.
+template <typename T, int P, int R>
+__device__ __forceinline__ void gather_carry(const T (&v)[P], T (&nl)[R], T (&nr)[R],
+                                             T (&e)[P + 2 * R])
+{
+    static_assert(P >= R, "a lane must hold at least one stencil radius");
+#pragma unroll
+    for (int p = 0; p < P; ++p) e[R + p] = v[p];
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        nl[i] = from_left(v[P - 1 - i], nl[i]);
+        nr[i] = from_right(v[i], nr[i]);
+        e[R - 1 - i] = nl[i];
+        e[R + P + i] = nr[i];
+    }
+}
+
 
 }  // namespace sw1d

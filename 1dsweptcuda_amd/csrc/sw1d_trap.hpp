@@ -59,10 +59,12 @@ __device__ __forceinline__ void trap_heat(const T* __restrict__ in, T* __restric
     int64_t g0 = 0;
     if (BND) g0 = wrapn(win + (int64_t)lane * PI, n);
     load_window<T, PI>(in, n, win, lane, v);
+    // The window's outer points are discarded, so lane 0 / lane 63 may see
+    // any finite value beyond it: the carried registers keep the first one.
+    T hl[1] = {v[0]}, hr[1] = {v[PI - 1]};
     for (int s = 0; s < S; ++s) {
         T e[PI + 2];
-        const T hl[1] = {v[0]}, hr[1] = {v[PI - 1]};
-        gather<T, PI, 1>(v, hl, hr, e);
+        gather_carry<T, PI, 1>(v, hl, hr, e);
 #pragma unroll
         for (int p = 0; p < PI; ++p) {
             T l = e[p], c = e[p + 1], r = e[p + 2];
@@ -87,18 +89,19 @@ __device__ __forceinline__ void trap_ks(const T* __restrict__ in, T* __restrict_
     constexpr int RS = 2 * S;
     T u[PI], uh[PI];
     load_window<T, PI>(in, n, win, lane, u);
+    // beyond the window (discarded points): the carried registers' values
+    T hl[2] = {u[0], u[0]}, hr[2] = {u[0], u[0]};
+    T hlh[2] = {u[0], u[0]}, hrh[2] = {u[0], u[0]};
     for (int s = 0; s < S; s += 2) {
         {
             T e[PI + 4];
-            const T hl[2] = {u[0], u[0]}, hr[2] = {u[0], u[0]};
-            gather<T, PI, 2>(u, hl, hr, e);
+            gather_carry<T, PI, 2>(u, hl, hr, e);
 #pragma unroll
             for (int p = 0; p < PI; ++p) uh[p] = ks_predict(e[p], e[p + 1], e[p + 2], e[p + 3], e[p + 4], k);
         }
         {
             T e[PI + 4];
-            const T hl[2] = {uh[0], uh[0]}, hr[2] = {uh[0], uh[0]};
-            gather<T, PI, 2>(uh, hl, hr, e);
+            gather_carry<T, PI, 2>(uh, hlh, hrh, e);
 #pragma unroll
             for (int p = 0; p < PI; ++p)
                 u[p] = ks_correct(u[p], e[p], e[p + 1], e[p + 2], e[p + 3], e[p + 4], k);
@@ -110,17 +113,13 @@ __device__ __forceinline__ void trap_ks(const T* __restrict__ in, T* __restrict_
 // ----------------------------------------------------------------- Euler ---
 template <typename T, int PI, bool BND>
 __device__ __forceinline__ void eu_trap_stage(const T (&src)[3][PI], T (&dst)[3][PI],
-                                              const T (&base)[3][PI], int64_t g0, int64_t n,
-                                              T c, const EulerCoef<T>& k)
+                                              const T (&base)[3][PI], T (&hl)[3][2], T (&hr)[3][2],
+                                              int64_t g0, int64_t n, T c, const EulerCoef<T>& k)
 {
     T ex[PI + 4], ey[PI + 4], ez[PI + 4];
-    {
-        const T hx[2] = {src[0][0], src[0][0]}, hy[2] = {src[1][0], src[1][0]};
-        const T hz[2] = {src[2][0], src[2][0]};
-        gather<T, PI, 2>(src[0], hx, hx, ex);
-        gather<T, PI, 2>(src[1], hy, hy, ey);
-        gather<T, PI, 2>(src[2], hz, hz, ez);
-    }
+    gather_carry<T, PI, 2>(src[0], hl[0], hr[0], ex);
+    gather_carry<T, PI, 2>(src[1], hl[1], hr[1], ey);
+    gather_carry<T, PI, 2>(src[2], hl[2], hr[2], ez);
     auto cell = [&](int j) { return Q3<T>{ex[j], ey[j], ez[j]}; };
     // global index of ext cell j (lane's point p is ext cell p + 2)
     auto gidx = [&](int j) {
@@ -174,9 +173,16 @@ __device__ __forceinline__ void trap_euler(const T* __restrict__ in, T* __restri
 #pragma unroll
     for (int c = 0; c < 3; ++c) load_window<T, PI>(in + c * n, n, win, lane, q[c]);
     const T ca = T(0.25) * k.dtdx, cb = T(0.5) * k.dtdx;
+    // beyond the window (discarded cells): the carried registers' values,
+    // a physical state so that the discarded cells stay finite
+    T hl[3][2], hr[3][2], hlh[3][2], hrh[3][2];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int i = 0; i < 2; ++i) hl[c][i] = hr[c][i] = hlh[c][i] = hrh[c][i] = q[c][0];
     for (int s = 0; s < S; s += 2) {
-        eu_trap_stage<T, PI, BND>(q, qh, q, g0, n, ca, k);   // predictor (Euler:307-350)
-        eu_trap_stage<T, PI, BND>(qh, q, q, g0, n, cb, k);   // corrector (Euler:353-389)
+        eu_trap_stage<T, PI, BND>(q, qh, q, hl, hr, g0, n, ca, k);     // predictor (Euler:307-350)
+        eu_trap_stage<T, PI, BND>(qh, q, q, hlh, hrh, g0, n, cb, k);   // corrector (Euler:353-389)
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) store_window<T, PI>(out + c * n, n, out0, count, lane, RS, q[c]);
