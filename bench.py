@@ -13,8 +13,9 @@ plan's stream during a second pass over the same steps; achieved = algorithmic
 FP64 flops per launch (40 flop per KS point-update x nX x W/8 steps) / that
 duration.
 
-The CPU baseline is the serial CPU oracle (one core) on a bounded sample of
-the same workload (same nX, fewer steps), timed on this host.
+The CPU baseline is the C oracle (the same KS stages) on a bounded sample of
+the same workload (same nX, fewer steps), timed on this host with OpenMP over
+the threads this job may use (at most 16) and with one thread.
 
 Prints one JSON line.
 """
@@ -39,7 +40,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per whole-diamond launch (nX, node width) -> bytes, from rocprofv3
 # FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by its reading for a copy
 # kernel of known size (profiles/ks_swept_diamond_nx1048576_hbm_bytes.md).
-DIAMOND_TRAFFIC_BYTES = {(1 << 20, 512): 33.96e6, (1 << 20, 1024): 34.28e6}
+DIAMOND_TRAFFIC_BYTES = {(1 << 20, 512): 33.96e6, (1 << 20, 1024): 34.75e6}
 
 
 def _timed(plan, steps):
@@ -54,7 +55,10 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20000)
-    ap.add_argument("--warmup", type=int, default=2000)
+    # 20000 untimed steps (~26 ms at 2^20): with 2000, the first timed steps of a
+    # fresh process still run with gaps between launches (same kernel times,
+    # ~5% lower wall-clock rate)
+    ap.add_argument("--warmup", type=int, default=20000)
     ap.add_argument("--nx", type=int, default=1 << 20)
     ap.add_argument("--node-width", type=int, default=1024)
     ap.add_argument("--seed", type=int, default=1)
