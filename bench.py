@@ -345,32 +345,55 @@ def _surrogate():
             "seconds": time.perf_counter() - t0}
 
 
-def _cpu_baseline(n, dt, u0, budget_s):
+def _cpu_baseline(n, dt, u0, budget_s, repeats=5):
     """The C oracle (the same KS stages, FP64) on this host: OpenMP over the
-    host threads this job may use (at most 16), and one thread; each on a
-    sample of about budget_s/2 seconds of the same nX."""
+    host threads this job may use (at most 16), and one thread.  For each,
+    the step count is set so that one run takes about budget_s / (2 repeats)
+    seconds (at least 1 s), and the median rate of `repeats` runs is kept;
+    only the oracle call is timed (the state copy into it is ~1 ms)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
 
     c = pyoracle.ks_consts(dt)
 
     def rate(threads, seconds):
-        t0 = time.perf_counter()
-        pyoracle.ks_run(u0, 2, c, threads)
-        per_step = (time.perf_counter() - t0) / 2
-        steps = max(2, int(seconds / max(per_step, 1e-9)))
-        t0 = time.perf_counter()
-        pyoracle.ks_run(u0, steps, c, threads)
-        return n * steps / (time.perf_counter() - t0), steps
+        pyoracle.ks_run(u0, 2, c, threads)  # thread pool start-up, untimed
+        steps, took = 2, 0.0
+        while took < 0.1:  # calibrate on a run of at least 0.1 s
+            steps *= 2
+            t0 = time.perf_counter()
+            pyoracle.ks_run(u0, steps, c, threads)
+            took = time.perf_counter() - t0
+        steps = max(2, int(seconds * steps / took))
+        rates = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            pyoracle.ks_run(u0, steps, c, threads)
+            rates.append(n * steps / (time.perf_counter() - t0))
+        return float(np.median(rates)), steps
 
     avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     threads = max(1, min(16, avail))
-    omp, omp_steps = rate(threads, budget_s / 2)
-    one, one_steps = rate(1, budget_s / 2)
+    per_run = max(1.0, budget_s / (2 * repeats))
+    omp, omp_steps = rate(threads, per_run)
+    one, one_steps = rate(1, per_run)
     return {"value": omp, "unit": "point-updates/s", "cores": threads, "kind": "port",
-            "sample": f"KS FP64 nX={n}, {omp_steps} steps, C oracle (oracle/swept1d_oracle.c) "
-                      f"with OpenMP over {threads} threads",
-            "serial": {"value": one, "cores": 1, "sample": f"same, {one_steps} steps, one thread"}}
+            "sample": f"KS FP64 nX={n}, {omp_steps} steps per run, median of {repeats} runs, "
+                      f"C oracle (oracle/swept1d_oracle.c) with OpenMP over {threads} threads",
+            "serial": {"value": one, "cores": 1,
+                       "sample": f"same, {one_steps} steps per run, median of {repeats}, one thread"},
+            "host": {"cpu": _cpu_model(), "cpus_visible": os.cpu_count(), "cpus_allowed": avail}}
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 if __name__ == "__main__":

@@ -12,3 +12,16 @@ def test_more_than_one_gpu_is_refused():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_cpu_baseline_fields():
+    """The CPU baseline entry: the oracle timed with OpenMP and with one
+    thread, median of the repeats, and the host it ran on."""
+    sys.path.insert(0, ROOT)
+    import bench
+
+    r = bench._cpu_baseline(4096, 0.005, bench._ks_ic(4096, 1), 0.5, repeats=3)
+    assert r["kind"] == "port" and r["unit"] == "point-updates/s"
+    assert r["value"] > 0 and r["serial"]["value"] > 0 and r["serial"]["cores"] == 1
+    assert 1 <= r["cores"] <= 16 and r["cores"] <= r["host"]["cpus_allowed"]
+    assert "median of 3" in r["sample"] and r["host"]["cpu"]
