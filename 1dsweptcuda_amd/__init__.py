@@ -311,6 +311,7 @@ class Surrogate:
         L.sw1d_nn_create.argtypes = [C.POINTER(P), C.c_int32, C.c_int32, C.c_uint64]
         L.sw1d_nn_destroy.argtypes = [P]
         L.sw1d_nn_set_scales.argtypes = [P, C.c_double, C.c_double]
+        L.sw1d_nn_set_inference.argtypes = [P, C.c_int32]
         L.sw1d_nn_get_params.argtypes = [P, P]
         L.sw1d_nn_set_params.argtypes = [P, P]
         L.sw1d_nn_forward.argtypes = [P, P, C.c_int32, P]
@@ -360,6 +361,12 @@ class Surrogate:
 
     def set_scales(self, in_scale: float, out_scale: float) -> None:
         self._chk(lib().sw1d_nn_set_scales(self._h, in_scale, out_scale), "sw1d_nn_set_scales")
+
+    # inference kernels (sw1d_nn_set_inference): all give the same bits
+    INFER_AUTO, INFER_PER_LAYER, INFER_WHOLE = 0, 1, 2
+
+    def set_inference(self, mode: int) -> None:
+        self._chk(lib().sw1d_nn_set_inference(self._h, int(mode)), "sw1d_nn_set_inference")
 
     def get_params(self) -> np.ndarray:
         p = np.empty(self.param_count(), np.float32)

@@ -430,6 +430,181 @@ __global__ __launch_bounds__(NT) void k_l6_out(const float* __restrict__ S, cons
         du[i] = s;
 }
 
+// parameter vector (FP32), in this order
+constexpr int64_t P_W1 = 0, P_B1 = P_W1 + C * TAPS;
+constexpr int64_t P_MID = P_B1 + C;                  // 4 x (W[t][co][ci], b[co])
+constexpr int64_t MID_SZ = TAPS * C * C + C;
+constexpr int64_t P_W6 = P_MID + 4 * MID_SZ, P_B6 = P_W6 + TAPS * C;
+constexpr int64_t NPARAM = P_B6 + 1;
+
+// ------------------------------------------ whole network, one sample per WG ---
+// Inference at nx = 1024: one workgroup of 16 waves holds a whole periodic
+// sample's activation (1024 rows x 64 BF16 = 128 KB) in LDS through all six
+// layers, so no activation goes to memory; each wave owns 64 rows (four
+// 16-row MFMA blocks).  The tile keeps copies of rows 1022, 1023 before row 0
+// and of rows 0, 1 after row 1023 (tile row = position + 2, as in the
+// per-layer kernels), refreshed after each layer, so the MFMA operand rows
+// need no wrap.  A layer's weights are staged in two parts (taps 0-2, then
+// 3-4) into a 24 KB buffer, which also holds the network input before layer 1
+// and the output layer's tap sums after layer 5.  Every value goes through
+// the operations of the per-layer kernels in the same order (L1 as stage_l1,
+// the MFMA steps s = 0..9 of mfma_tile, the epilogue, the tap sums of
+// k_l56_fwd and the sum of k_l6_out), so the results are the same bits.
+constexpr int XN = 1024;                   // positions per sample
+constexpr int NTN = 1024;                  // threads of the whole-network kernel
+constexpr int MBN = XN / (NTN / 64) / 16;  // 16-row blocks per wave (4)
+constexpr int WPART = 3;                   // taps per weight part
+static_assert(NTN == XN, "the input and output stages map one thread to one position");
+
+// taps t0 .. t0+nt-1 of a layer's packed weights [t][co][ci] -> sW rows (t-t0)*64 + co
+__device__ __forceinline__ void stage_w_part(__bf16* sW, const __bf16* __restrict__ Wf, int t0, int nt)
+{
+    for (int q = threadIdx.x; q < nt * C * (C / 8); q += NTN) {
+        *reinterpret_cast<uint4*>(&sW[swz(q / (C / 8), q % (C / 8))]) =
+            *reinterpret_cast<const uint4*>(&Wf[(int64_t)t0 * C * C + q * 8]);
+    }
+}
+// tile rows 0, 1 <- rows 1024, 1025 and rows 1026, 1027 <- rows 2, 3 (the
+// periodic neighbours of positions 0 and 1023)
+__device__ __forceinline__ void wrap_rows(__bf16* sA)
+{
+    const int q = threadIdx.x;
+    if (q < 4 * (C / 8)) {
+        const int d = q / (C / 8), c8 = q % (C / 8);
+        const int dst = d < 2 ? d : XN + d, src = d < 2 ? XN + d : d;
+        *reinterpret_cast<uint4*>(&sA[swz(dst, c8)]) = *reinterpret_cast<const uint4*>(&sA[swz(src, c8)]);
+    }
+}
+// MFMA steps s = S0 .. S1-1 of mfma_tile over the wave's 64 rows (tile rows
+// row0 .. row0+63 plus the four halo rows); the weights of tap t sit at sW
+// rows (t - S0/2)*64 + co.
+template <int S0, int S1>
+__device__ __forceinline__ void mfma_rows(const __bf16* sA, const __bf16* sW, int row0, f32x4 (&acc)[MBN][4])
+{
+    const int l = threadIdx.x & 63, li = l % 16, lq = l / 16;
+    for (int s = S0; s < S1; ++s) {
+        const int t = s >> 1, chunk = (s & 1) * 4 + lq, tw = t - S0 / 2;
+        const __bf16* pa = &sA[swz(row0 + li + t, chunk)];      // + 16 rows per block
+        const __bf16* pb = &sW[swz(tw * C + li, chunk)];        // + 16 rows per block
+        bf16x8 a[MBN], bv[4];
+#pragma unroll
+        for (int mb = 0; mb < MBN; ++mb) a[mb] = *reinterpret_cast<const bf16x8*>(pa + 16 * mb * C);
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb) bv[nb] = *reinterpret_cast<const bf16x8*>(pb + 16 * nb * C);
+#pragma unroll
+        for (int mb = 0; mb < MBN; ++mb)
+#pragma unroll
+            for (int nb = 0; nb < 4; ++nb)
+                acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mb], bv[nb], acc[mb][nb], 0, 0, 0);
+    }
+}
+
+struct MidWeights { const __bf16* w[4]; };  // packed forward weights of L2-L5
+
+// TIN = double: in = FP64 states u (normalised by inv_scale), u += out_scale *
+// du in place (in and u are the same array); TIN = float: in = network
+// inputs x, du written to du.
+template <typename TIN>
+__global__ __launch_bounds__(NTN) void k_net_fwd(const TIN* in, float inv_scale, const float* __restrict__ P,
+                                                MidWeights wf, float* __restrict__ du, double* u,
+                                                double out_scale)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 sA[(XN + 4) * C];
+    __shared__ __attribute__((aligned(16))) __bf16 sW[WPART * C * C];
+    __shared__ float sW6[TAPS * C];
+    float* sF = reinterpret_cast<float*>(sW);  // network input, later the output-layer tap sums
+    const int b = blockIdx.x, r = threadIdx.x;
+    const int64_t o = (int64_t)b * XN + r;
+
+    // layer 1 (stage_l1 over the whole periodic sample) into tile rows 2 .. XN+1
+    sF[r] = (float)in[o] * inv_scale;
+    {
+        const int c8 = threadIdx.x % (C / 8);
+        float w[8][TAPS], bb[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            bb[k] = P[P_B1 + c8 * 8 + k];
+#pragma unroll
+            for (int t = 0; t < TAPS; ++t) w[k][t] = P[P_W1 + (c8 * 8 + k) * TAPS + t];
+        }
+        __syncthreads();
+        for (int x = threadIdx.x / (C / 8); x < XN; x += NTN / (C / 8)) {
+            bf16x8 v;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                float z = bb[k];
+#pragma unroll
+                for (int t = 0; t < TAPS; ++t) z = fmaf(w[k][t], sF[(x + t - 2) & (XN - 1)], z);
+                v[k] = f2bf(gelu(z));
+            }
+            *reinterpret_cast<bf16x8*>(&sA[swz(x + 2, c8)]) = v;
+        }
+    }
+    for (int q = threadIdx.x; q < TAPS * C; q += NTN) sW6[q] = P[P_W6 + q];
+
+    // layers 2-5
+    for (int layer = 0; layer < 4; ++layer) {
+        // recomputed per layer rather than kept live across it
+        int lane = threadIdx.x;
+        asm volatile("" : "+v"(lane));
+        const int l = lane & 63, row0 = 64 * (lane >> 6), li = l % 16, lq = l / 16;
+        f32x4 acc[MBN][4];
+#pragma unroll
+        for (int mb = 0; mb < MBN; ++mb)
+#pragma unroll
+            for (int nb = 0; nb < 4; ++nb) acc[mb][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+        __syncthreads();  // the layer input is complete; sF / sW are no longer read
+        wrap_rows(sA);
+        stage_w_part(sW, wf.w[layer], 0, WPART);
+        __syncthreads();
+        mfma_rows<0, 2 * WPART>(sA, sW, row0, acc);
+        __syncthreads();
+        stage_w_part(sW, wf.w[layer], WPART, TAPS - WPART);
+        __syncthreads();
+        mfma_rows<2 * WPART, 2 * TAPS>(sA, sW, row0, acc);
+        __syncthreads();  // every wave has read the layer input: overwrite it in place
+        const float* bias = P + P_MID + layer * MID_SZ + TAPS * C * C;
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb) {
+            const int co = 16 * nb + li;
+            const float bc = bias[co];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                __bf16* pd = &sA[swz(row0 + 2 + 4 * lq + j, co >> 3) + (co & 7)];  // + 16 rows per block
+#pragma unroll
+                for (int mb = 0; mb < MBN; ++mb) pd[16 * mb * C] = f2bf(gelu(acc[mb][nb][j] + bc));
+            }
+        }
+    }
+    __syncthreads();
+
+    // output layer: the five tap sums of position r (k_l56_fwd), then
+    // du = b6 + sum_t S[t][r+t-2] (k_l6_out)
+    {
+        float sum[TAPS] = {0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int c8 = 0; c8 < C / 8; ++c8) {
+            const bf16x8 v = *reinterpret_cast<const bf16x8*>(&sA[swz(r + 2, c8)]);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const float a = bf2f(v[k]);
+#pragma unroll
+                for (int t = 0; t < TAPS; ++t) sum[t] = fmaf(sW6[t * C + c8 * 8 + k], a, sum[t]);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < TAPS; ++t) sF[t * XN + r] = sum[t];
+    }
+    __syncthreads();
+    float s = P[P_B6];
+#pragma unroll
+    for (int t = 0; t < TAPS; ++t) s += sF[t * XN + ((r + t - 2) & (XN - 1))];
+    if (u)
+        u[o] += out_scale * (double)s;
+    else
+        du[o] = s;
+}
+
 // dA[b][y][ci] = sum_t sum_co dZ[b][y-t+2][co] W[t][co][ci]   (W as Wb[t][ci][co])
 // then dZprev = bf16(dA * gelu'(Zprev)).
 __global__ __launch_bounds__(NT) void k_mid_bwd_data(const __bf16* __restrict__ dZ, const __bf16* __restrict__ Wb,
@@ -729,13 +904,6 @@ __global__ __launch_bounds__(NT) void k_l2_parts(const double* __restrict__ a, c
 // ================================================================== host ===
 namespace sw1dnn {
 
-// parameter vector (FP32), in this order
-constexpr int64_t P_W1 = 0, P_B1 = P_W1 + C * TAPS;
-constexpr int64_t P_MID = P_B1 + C;                  // 4 x (W[t][co][ci], b[co])
-constexpr int64_t MID_SZ = TAPS * C * C + C;
-constexpr int64_t P_W6 = P_MID + 4 * MID_SZ, P_B6 = P_W6 + TAPS * C;
-constexpr int64_t NPARAM = P_B6 + 1;
-
 inline unsigned nblk(int64_t n) { return (unsigned)((n + NT - 1) / NT); }
 constexpr int64_t RED_GROUPS = 32;  // first-pass groups of the gradient reduction
 
@@ -775,6 +943,7 @@ int g_mfma_ok = 0;
 
 struct sw1d_nn {
     int X = 1024, Bmax = 0;
+    int infer_mode = 0;     // 0 auto, 1 per-layer launches, 2 whole-network kernel
     double in_scale = 1.0, out_scale = 1.0;
     hipStream_t stream = nullptr;
     float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;  // NPARAM each
@@ -840,12 +1009,25 @@ int nn_forward(sw1d_nn* N, int B, bool store_z = true)
 
 // Inference of B samples.  u == nullptr: from the FP32 inputs in N->x to du
 // in N->du.  Otherwise from the FP64 states u (normalised on the fly by
-// 1/in_scale) to u += out_scale * du.  Five launches when nx is a multiple of
-// the 256-position tile; otherwise the training kernels plus the conversions.
+// 1/in_scale) to u += out_scale * du.  One launch of the whole-network kernel
+// at nx = 1024; five launches when nx is another multiple of the 256-position
+// tile (or infer_mode = 1); otherwise the training kernels plus the
+// conversions.  All give the same bits.
 int nn_infer(sw1d_nn* N, int B, double* u)
 {
     const int X = N->X;
     const int64_t plane = (int64_t)B * X;
+    if (X == XN && N->infer_mode != 1) {
+        MidWeights w{{N->wf[0], N->wf[1], N->wf[2], N->wf[3]}};
+        if (u)
+            hipLaunchKernelGGL(k_net_fwd<double>, dim3(B), dim3(NTN), 0, N->stream, u,
+                               (float)(1.0 / N->in_scale), N->p, w, nullptr, u, N->out_scale);
+        else
+            hipLaunchKernelGGL(k_net_fwd<float>, dim3(B), dim3(NTN), 0, N->stream, N->x, 1.0f, N->p, w, N->du,
+                               nullptr, 0.0);
+        NN_HIP(hipGetLastError());
+        return SW1D_OK;
+    }
     if (X % TILE_F != 0) {
         if (u)
             hipLaunchKernelGGL(k_to_input, dim3(nblk(plane)), dim3(NT), 0, N->stream, u, N->x, plane,
@@ -1062,6 +1244,16 @@ int sw1d_nn_set_scales(sw1d_nn* N, double in_scale, double out_scale)
     if (!N || !(in_scale > 0) || !(out_scale > 0)) return nn_fail(SW1D_EINVAL, "scales must be positive");
     N->in_scale = in_scale;
     N->out_scale = out_scale;
+    return SW1D_OK;
+}
+
+int sw1d_nn_set_inference(sw1d_nn* N, int32_t mode)
+{
+    if (!N || mode < 0 || mode > 2) return nn_fail(SW1D_EINVAL, "inference mode must be 0, 1 or 2");
+    if (mode == 2 && N->X != XN) return nn_fail(SW1D_EINVAL, "the whole-network kernel needs nx = 1024");
+    if (mode == 1 && N->X % TILE_F != 0)
+        return nn_fail(SW1D_EINVAL, "the per-layer inference kernels need nx to be a multiple of 256");
+    N->infer_mode = mode;
     return SW1D_OK;
 }
 

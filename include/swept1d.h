@@ -178,6 +178,12 @@ int sw1d_nn_device_check(void);
 int sw1d_nn_create(sw1d_nn** nn, int32_t nx, int32_t max_batch, uint64_t seed);
 int sw1d_nn_destroy(sw1d_nn* nn);
 int sw1d_nn_set_scales(sw1d_nn* nn, double in_scale, double out_scale);
+/* Inference kernels (forward and surrogate steps): 0 = automatic (one
+ * whole-network launch at nx = 1024, per-layer launches for other multiples
+ * of 256, the training kernels otherwise), 1 = per-layer launches (nx a
+ * multiple of 256), 2 = the whole-network kernel (nx = 1024).  Every choice
+ * gives the same bits. */
+int sw1d_nn_set_inference(sw1d_nn* nn, int32_t mode);
 int sw1d_nn_get_params(sw1d_nn* nn, float* host);
 int sw1d_nn_set_params(sw1d_nn* nn, const float* host);
 int sw1d_nn_forward(sw1d_nn* nn, const float* x_host, int32_t batch, float* y_host);

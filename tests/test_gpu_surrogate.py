@@ -82,6 +82,45 @@ def test_network_step_is_forward_plus_update(sw):
     assert np.array_equal(u1, want), float(np.abs(u1 - want).max())
 
 
+def test_whole_network_kernel_matches_per_layer_launches(sw):
+    """At nx = 1024 inference is one whole-network launch (the activation
+    resident in LDS); the five per-layer launches give the same bits, for the
+    forward entry point and for a network step on FP64 states."""
+    import importlib
+
+    wl = importlib.import_module("1dsweptcuda_amd.workloads")
+    B, nx = 5, 1024
+    x = np.random.default_rng(11).standard_normal((B, nx)).astype(np.float32)
+    u0 = np.concatenate([wl.fourier_ic(nx, nx * 0.5, 60 + i) for i in range(B)])
+    p = _params(12)
+    out = {}
+    with sw.Surrogate(nx, B, seed=0) as nn:
+        nn.set_params(p)
+        nn.set_scales(1.3, 0.07)
+        for mode in (nn.INFER_WHOLE, nn.INFER_PER_LAYER):
+            print("surrogate inference mode", mode, flush=True)
+            nn.set_inference(mode)
+            y = nn.forward(x)
+            with sw.Batch(np.ones(B, np.int32), np.full(B, nx, np.int64), np.full(B, 0.005),
+                          np.full(B, 0.5)) as b:
+                b.set_state(u0)
+                nn.advance(b, 3, 20, judge_every=0)
+                out[mode] = (y, b.get_state())
+    assert np.array_equal(out[1][0], out[2][0])
+    assert np.array_equal(out[1][1], out[2][1])
+    assert not np.array_equal(out[2][1], u0)
+
+
+def test_inference_mode_validation(sw):
+    with sw.Surrogate(512, 2, seed=0) as nn:
+        with pytest.raises(sw.SweptError):
+            nn.set_inference(nn.INFER_WHOLE)  # the whole-network kernel is for nx = 1024
+        nn.set_inference(nn.INFER_PER_LAYER)
+    with sw.Surrogate(192, 2, seed=0) as nn:
+        with pytest.raises(sw.SweptError):
+            nn.set_inference(nn.INFER_PER_LAYER)  # not a multiple of 256
+
+
 def test_gradient_matches_oracle(sw):
     rng = np.random.default_rng(3)
     x = rng.standard_normal((2, 1024)).astype(np.float32)
