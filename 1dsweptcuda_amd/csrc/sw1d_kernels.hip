@@ -191,8 +191,8 @@ __device__ __forceinline__ void wave_sync()
 
 // Swept heat tiles: one wave per node of W = 64*P points.  For P >= 2 the
 // node's image sits in LDS and the diamond's levels run on centred regions of
-// 1 .. P (P <= 4) or 2, 4, .. P (P >= 8) points per lane, each level on the
-// narrowest region holding its active set and the one-point stencil reach:
+// 1, 2, .. P points per lane, each level on the narrowest region holding its
+// active set and the one-point stencil reach:
 // the same operations, so the same bits, on fewer points.
 template <typename T, int P, int TILE, bool MIRROR>
 __global__ __launch_bounds__(kBlock) void k_heat_swept(const T* __restrict__ in,
@@ -223,11 +223,10 @@ __global__ __launch_bounds__(kBlock) void k_heat_swept(const T* __restrict__ in,
             full.stage(W - 1 - 2 * (H - 1), full.r0, full.hl0, full.hr0, full.r1, fo, fo2);
         }
     } else {
-        // Regions of PC = STEP, 2 STEP, .. P - STEP points per lane (widths
-        // 64 PC): region PC takes down pairs 32(PC-STEP)+1 <= s <= 32 PC - 1 and
-        // up pairs 32(P-PC)+1 <= m <= 32(P-PC+STEP)-1 (to H-3 for the
-        // narrowest), where every threshold is <= 64 PC - 1.
-        constexpr int STEP = P >= 8 ? 2 : 1;
+        // Regions of PC = 1, 2, .. P - 1 points per lane (widths 64 PC):
+        // region PC takes down pairs 32(PC-1)+1 <= s <= 32 PC - 1 and up pairs
+        // 32(P-PC)+1 <= m <= 32(P-PC+1)-1 (to H-3 for PC = 1), where every
+        // threshold is <= 64 PC - 1.
         T* img = img_all + (threadIdx.x >> 6) * 2 * W;
         const int lane = g.lane;
         auto reload_full = [&] {
@@ -241,7 +240,7 @@ __global__ __launch_bounds__(kBlock) void k_heat_swept(const T* __restrict__ in,
             constexpr int PC = decltype(pc_tag)::value;
             HeatRegion<T, PC, MIRROR> r;
             r.template load<W>(img, lane, g.base, n);
-            r.down(32 * (PC - STEP) + 1, 32 * PC - 1, fo, fo2);
+            r.down(32 * (PC - 1) + 1, 32 * PC - 1, fo, fo2);
             r.template save<W>(img, lane);
             wave_sync();
         };
@@ -249,29 +248,29 @@ __global__ __launch_bounds__(kBlock) void k_heat_swept(const T* __restrict__ in,
             constexpr int PC = decltype(pc_tag)::value;
             HeatRegion<T, PC, MIRROR> r;
             r.template load<W>(img, lane, g.base, n);
-            if constexpr (PC == STEP) {
+            if constexpr (PC == 1) {
                 r.up(W, 32 * (P - PC) + 1, H - 3, fo, fo2);
                 r.stage(W - 1 - 2 * (H - 1), r.r0, r.hl0, r.hr0, r.r1, fo, fo2);
             } else {
-                r.up(W, 32 * (P - PC) + 1, 32 * (P - PC + STEP) - 1, fo, fo2);
+                r.up(W, 32 * (P - PC) + 1, 32 * (P - PC + 1) - 1, fo, fo2);
             }
             r.template save<W>(img, lane);
             wave_sync();
         };
-        using Regions = std::make_integer_sequence<int, P / STEP - 1>;  // i -> PC = STEP (i + 1)
+        using Regions = std::make_integer_sequence<int, P - 1>;  // i -> PC = i + 1
         if (TILE != TILE_UP) {
             full.template save<W>(img, lane);
             wave_sync();
-            for_each_index([&](auto i) { down_region(std::integral_constant<int, STEP * (decltype(i)::value + 1)>{}); },
+            for_each_index([&](auto i) { down_region(std::integral_constant<int, decltype(i)::value + 1>{}); },
                            Regions{});
             reload_full();
-            full.down(32 * (P - STEP) + 1, H - 1, fo, fo2);
+            full.down(32 * (P - 1) + 1, H - 1, fo, fo2);
         }
         if (TILE != TILE_DOWN) {
-            full.up(W, 1, 32 * STEP - 1, fo, fo2);
+            full.up(W, 1, 31, fo, fo2);
             full.template save<W>(img, lane);
             wave_sync();
-            for_each_index([&](auto i) { up_region(std::integral_constant<int, P - STEP * (decltype(i)::value + 1)>{}); },
+            for_each_index([&](auto i) { up_region(std::integral_constant<int, P - 1 - decltype(i)::value>{}); },
                            Regions{});
             reload_full();
         }

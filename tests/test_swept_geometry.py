@@ -78,9 +78,9 @@ def test_ks_region_phases_contain_active_sets(P):
 def test_heat_region_phases_contain_active_sets(P):
     """Same check for the heat diamond kernel (k_heat_swept, P >= 2): one-point
     stencil, thresholds 2s-1, 2s+1 down and W-1-2m, W-3-2m up; regions of
-    1 .. P-1 (P <= 4) or 2, 4, .. P-2 (P >= 8) points per lane."""
+    1, 2, .. P-1 points per lane."""
     W, H = 64 * P, 32 * P
-    step = 2 if P >= 8 else 1
+    step = 1
     pcs = list(range(step, P, step))
     down = lambda s0, s1: [t for s in range(s0, s1 + 1, 2) for t in (2 * s - 1, 2 * s + 1)]
     up = lambda m0, m1: [t for m in range(m0, m1 + 1, 2) for t in (W - 1 - 2 * m, W - 3 - 2 * m)]
