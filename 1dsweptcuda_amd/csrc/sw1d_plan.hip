@@ -513,6 +513,52 @@ void release(sw1d_plan* p)
 
 int g_lane_check = 0;  // 0 unknown, 1 pass, -1 fail
 
+// Euler states: callers hold interleaved (rho, m, E) cells (the reference's
+// REALthree arrays), the plan three component arrays plus their predictor
+// copies.  Bit copies of 4- or 8-byte words, one thread per cell.
+template <typename U>
+__global__ __launch_bounds__(256) void k_split3(const U* __restrict__ in, U* __restrict__ out, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const U v = in[3 * i + c];
+        out[c * n + i] = v;
+        out[(c + 3) * n + i] = v;  // the predictor half starts as a copy
+    }
+}
+template <typename U>
+__global__ __launch_bounds__(256) void k_join3(const U* __restrict__ in, U* __restrict__ out, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[3 * i + c] = in[c * n + i];
+}
+
+hipError_t euler_layout(sw1d_plan* p, const void* in, void* out, bool split)
+{
+    const int64_t n = p->n;
+    const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+    if (p->S == 8) {
+        if (split)
+            hipLaunchKernelGGL(k_split3<uint64_t>, grid, block, 0, p->stream, static_cast<const uint64_t*>(in),
+                               static_cast<uint64_t*>(out), n);
+        else
+            hipLaunchKernelGGL(k_join3<uint64_t>, grid, block, 0, p->stream, static_cast<const uint64_t*>(in),
+                               static_cast<uint64_t*>(out), n);
+    } else {
+        if (split)
+            hipLaunchKernelGGL(k_split3<uint32_t>, grid, block, 0, p->stream, static_cast<const uint32_t*>(in),
+                               static_cast<uint32_t*>(out), n);
+        else
+            hipLaunchKernelGGL(k_join3<uint32_t>, grid, block, 0, p->stream, static_cast<const uint32_t*>(in),
+                               static_cast<uint32_t*>(out), n);
+    }
+    return hipGetLastError();
+}
+
 }  // namespace
 
 // ================================================================= C ABI ===
@@ -685,18 +731,21 @@ static int copy_state(sw1d_plan* p, const void* src, void* dst, bool to_device, 
             HIP_OR_FAIL(hipMemcpyAsync(dst, dev, n * S, kind, p->stream));
         }
     } else {
-        // interleaved (rho, m, E) cells <-> three component arrays
-        for (int c = 0; c < 3; ++c) {
-            if (to_device) {
-                HIP_OR_FAIL(hipMemcpy2DAsync(dev + (size_t)c * n * S, S,
-                                             static_cast<const char*>(src) + c * S, 3 * S, S, n,
-                                             kind, p->stream));
-                HIP_OR_FAIL(hipMemcpyAsync(dev + (size_t)(c + 3) * n * S, dev + (size_t)c * n * S,
-                                           n * S, hipMemcpyDeviceToDevice, p->stream));
-            } else {
-                HIP_OR_FAIL(hipMemcpy2DAsync(static_cast<char*>(dst) + c * S, 3 * S,
-                                             dev + (size_t)c * n * S, S, S, n, kind, p->stream));
+        // Host buffers go through the plan's other state buffer (idle between
+        // advances, 6n values) in one contiguous copy; device buffers are
+        // converted in place of the copy.
+        const bool host = kind != hipMemcpyDeviceToDevice;
+        void* scratch = p->buf[p->cur ^ 1];
+        if (to_device) {
+            const void* in = src;
+            if (host) {
+                HIP_OR_FAIL(hipMemcpyAsync(scratch, src, 3 * n * S, kind, p->stream));
+                in = scratch;
             }
+            HIP_OR_FAIL(euler_layout(p, in, dev, true));
+        } else {
+            HIP_OR_FAIL(euler_layout(p, dev, host ? scratch : dst, false));
+            if (host) HIP_OR_FAIL(hipMemcpyAsync(dst, scratch, 3 * n * S, kind, p->stream));
         }
     }
     HIP_OR_FAIL(hipStreamSynchronize(p->stream));
@@ -901,6 +950,7 @@ void batch_release(sw1d_batch* B)
     if (B->stream) (void)hipStreamDestroy(B->stream);
     delete B;
 }
+
 
 }  // namespace
 

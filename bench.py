@@ -43,6 +43,14 @@ HBM_PEAK_GBS = 8000.0
 DIAMOND_TRAFFIC_BYTES = {(1 << 20, 512): 33.96e6, (1 << 20, 1024): 34.75e6}
 
 
+_T0 = time.perf_counter()
+
+
+def _progress(section):
+    """One line per finished section on stderr (stdout carries only the JSON line)."""
+    print(f"[bench] {section} done at {time.perf_counter() - _T0:.1f} s", file=sys.stderr, flush=True)
+
+
 def _timed(plan, steps):
     plan.synchronize()
     t0 = time.perf_counter()
@@ -107,6 +115,7 @@ def main():
     flops = KS_FLOP_PER_PU * pu_per_launch
     achieved_tf = flops / (avg_ms * 1e-3) / 1e12
     plan.close()
+    _progress("swept")
 
     # ------------------------------------------------------- classic (alg 0)
     cplan = sw.Plan(sw.KS, n, dt, algorithm=sw.CLASSIC, tpb=256)
@@ -120,6 +129,7 @@ def main():
     # (both stage launches and anything between them): predictor reads u and
     # writes u_half (16 B), corrector reads u_half and u and writes u (24 B).
     classic_gbs = 40 * n * csteps / csecs / 1e9
+    _progress("classic")
 
     out = {
         "metric": "point-updates/s (KS FP64 swept, nX=2^20, bitwise equal to the FP64 CPU oracle)",
@@ -152,18 +162,18 @@ def main():
                      "algorithmic_bytes_per_launch": 32 * n},
     }
     extras = set() if args.no_extras else set(args.extras.split(",")) - {"none", ""}
-    if "trapezoid" in extras:
-        out["trapezoid"] = _trapezoid(sw, n, dt, rng_ic, args)
-    if "equations" in extras:
-        out["equations"] = _equations(sw, n)
-    if "sizes" in extras:
-        out["sizes"] = _sizes(sw, args.seed)
-    if "batched" in extras:
-        out["batched"] = _batched(sw)
-    if "surrogate" in extras:
-        out["surrogate"] = _surrogate()
+    sections = (("trapezoid", lambda: _trapezoid(sw, n, dt, rng_ic, args)),
+                ("equations", lambda: _equations(sw, n)),
+                ("sizes", lambda: _sizes(sw, args.seed)),
+                ("batched", lambda: _batched(sw)),
+                ("surrogate", _surrogate))
+    for name, run in sections:
+        if name in extras:
+            out[name] = run()
+            _progress(name)
     if not args.no_cpu:
         out["cpu_baseline"] = _cpu_baseline(n, dt, rng_ic, args.cpu_seconds)
+        _progress("cpu_baseline")
     print(json.dumps(out))
 
 
@@ -215,7 +225,7 @@ def _sizes(sw, seed, steps=20000):
     """The KS grid of the reference's performance sweeps, FP64: nX = 2^14 ..
     2^20 with swept node widths 128 .. 1024 (the reference's tpb), the
     classic algorithm and the register variant (alg 2), 20000 steps each;
-    and FP32 at 2^20."""
+    Euler (Sod) classic and swept over the same sizes; and KS FP32 at 2^20."""
     res = {}
     for lg in (14, 16, 18, 20):
         n = 1 << lg
@@ -237,6 +247,31 @@ def _sizes(sw, seed, steps=20000):
         row["best_swept"] = {"node_width": int(best), "value": row["swept"][best]}
         row["best_swept_over_classic"] = row["swept"][best] / row["classic"]
         res["2^%d" % lg] = row
+        _progress(f"sizes KS 2^{lg}")
+    # Euler (Sod, dt/dx = 0.1, FP64) over the same sizes: classic against the
+    # swept node widths its LDS image allows, 2000 steps each
+    eu = {}
+    for lg in (14, 16, 18, 20):
+        n = 1 << lg
+        q = np.zeros((n, 3))
+        q[: n // 2] = (1.0, 0.0, 1.0 / 0.4)
+        q[n // 2:] = (0.125, 0.0, 0.1 / 0.4)
+        dt_eu = 0.1 / (n - 2)
+        row = {"swept": {}}
+        for alg, W in ((sw.CLASSIC, 256), (sw.SWEPT, 256), (sw.SWEPT, 512)):
+            p = sw.Plan(sw.EULER, n, dt_eu, algorithm=alg, tpb=W, node_width=W if alg == sw.SWEPT else 0)
+            p.set_state(q.ravel())
+            p.advance(200)
+            secs = _timed(p, 2000)
+            p.close()
+            if alg == sw.SWEPT:
+                row["swept"][str(W)] = n * 2000 / secs
+            else:
+                row["classic"] = n * 2000 / secs
+        best = max(row["swept"], key=row["swept"].get)
+        row["best_swept_over_classic"] = row["swept"][best] / row["classic"]
+        eu["2^%d" % lg] = row
+    res["euler_f64"] = eu
     # FP32 at the largest size: classic and the swept node widths that can win there
     n = 1 << 20
     u0 = _ks_ic(n, seed).astype(np.float32)
