@@ -311,72 +311,52 @@ template <typename T>
 void cpu_node_diamond(sw1d_plan* p, T* win)
 {
     const int W = p->W, R = p->R, WW = W + 2 * R, H = W / (2 * R);
-    const int64_t n = p->n;
     // node-local x = 0 is global n - W/2: global n-1 at x = W/2-1, global 0 at x = W/2.
     const int xg0 = W / 2, xgN = W / 2 - 1;
-    std::vector<T> nv((size_t)WW * 3);
-    auto at = [&](int c, int x) -> T& { return win[(size_t)c * WW + (x + R)]; };
-    auto dist = [&](int x) { return std::abs(2 * x - (W - 1)); };
-    (void)n;
+    // component c of the window, indexed by node-local x (halo x = -R .. -1, W .. W+R-1)
+    auto row = [&](int c) { return win + (size_t)c * WW + R; };
 
+    // A stage updates its active set |2x - (W-1)| <= thr, the points x = lo .. W-1-lo
+    // (thr is odd, W even), and keeps the others.  Sources and destinations are
+    // different rows, or the same point only (the corrector's base), so each
+    // stage writes its destination row directly.
     auto stage = [&](int thr, int st) {
+        const int lo = std::max(0, (W - 1 - thr) / 2), hi = W - 1 - lo;
         if (p->eq == EQ_HEAT) {
             const T fo = (T)(p->f64 ? p->coef.fo_d : p->coef.fo_f);
             const T fo2 = (T)(p->f64 ? p->coef.fo2_d : p->coef.fo2_f);
-            const int src = st == 0 ? 0 : 1, dst = 1 - src;
-            for (int x = 0; x < W; ++x) {
-                if (dist(x) > thr) continue;
-                T l = at(src, x - 1), c = at(src, x), r = at(src, x + 1);
-                if (p->bc == BC_MIRROR) {
-                    if (x == xg0) l = r;
-                    if (x == xgN) r = l;
-                }
-                nv[x] = heat_ftcs(l, c, r, fo, fo2);
+            const T* __restrict__ s = row(st == 0 ? 0 : 1);
+            T* __restrict__ d = row(st == 0 ? 1 : 0);
+            for (int x = lo; x <= hi; ++x) d[x] = heat_ftcs(s[x - 1], s[x], s[x + 1], fo, fo2);
+            if (p->bc == BC_MIRROR) {  // insulated ends (Heat:140-142)
+                if (lo <= xg0 && xg0 <= hi) d[xg0] = heat_ftcs(s[xg0 + 1], s[xg0], s[xg0 + 1], fo, fo2);
+                if (lo <= xgN && xgN <= hi) d[xgN] = heat_ftcs(s[xgN - 1], s[xgN], s[xgN - 1], fo, fo2);
             }
-            for (int x = 0; x < W; ++x)
-                if (dist(x) <= thr) at(dst, x) = nv[x];
-        } else if (p->eq == EQ_KS) {
-            const KsCoef<T>& k = *reinterpret_cast<const KsCoef<T>*>(
-                p->f64 ? (const void*)&p->coef.ks_d : (const void*)&p->coef.ks_f);
-            const int src = st == 0 ? 0 : 1;
-            for (int x = 0; x < W; ++x) {
-                if (dist(x) > thr) continue;
-                const T a = at(src, x - 2), b = at(src, x - 1), c = at(src, x), d = at(src, x + 1),
-                        e = at(src, x + 2);
-                nv[x] = st == 0 ? ks_predict(a, b, c, d, e, k) : ks_correct(at(0, x), a, b, c, d, e, k);
-            }
-            const int dst = st == 0 ? 1 : 0;
-            for (int x = 0; x < W; ++x)
-                if (dist(x) <= thr) at(dst, x) = nv[x];
         } else {
             const EulerCoef<T>& k = *reinterpret_cast<const EulerCoef<T>*>(
                 p->f64 ? (const void*)&p->coef.eu_d : (const void*)&p->coef.eu_f);
             const int src = st == 0 ? 0 : 3, dst = 3 - src;
-            for (int x = 0; x < W; ++x) {
-                if (dist(x) > thr) continue;
-                Q3<T> s[5];
-                for (int j = 0; j < 5; ++j)
-                    s[j] = Q3<T>{at(src, x - 2 + j), at(src + 1, x - 2 + j), at(src + 2, x - 2 + j)};
-                const bool fl = (x == xg0 + 1), fr = (x == xgN - 1);
+            const T *s0 = row(src), *s1 = row(src + 1), *s2 = row(src + 2);
+            T *d0 = row(dst), *d1 = row(dst + 1), *d2 = row(dst + 2);
+            for (int x = lo; x <= hi; ++x) {
                 Q3<T> q;
                 if (x == xg0) {
                     q = k.bl;
                 } else if (x == xgN) {
                     q = k.br;
-                } else if (st == 0) {
-                    q = eu_predict(s[0], s[1], s[2], s[3], s[4], fl, fr, k);
                 } else {
-                    const Q3<T> u{at(0, x), at(1, x), at(2, x)};
-                    q = eu_correct(u, s[0], s[1], s[2], s[3], s[4], fl, fr, k);
+                    Q3<T> c[5];
+                    for (int j = 0; j < 5; ++j) c[j] = Q3<T>{s0[x - 2 + j], s1[x - 2 + j], s2[x - 2 + j]};
+                    const bool fl = x == xg0 + 1, fr = x == xgN - 1;
+                    if (st == 0)
+                        q = eu_predict(c[0], c[1], c[2], c[3], c[4], fl, fr, k);
+                    else
+                        q = eu_correct(Q3<T>{d0[x], d1[x], d2[x]}, c[0], c[1], c[2], c[3], c[4], fl, fr, k);
                 }
-                nv[3 * x + 0] = q.x; nv[3 * x + 1] = q.y; nv[3 * x + 2] = q.z;
+                d0[x] = q.x;
+                d1[x] = q.y;
+                d2[x] = q.z;
             }
-            for (int x = 0; x < W; ++x)
-                if (dist(x) <= thr) {
-                    at(dst, x) = nv[3 * x + 0];
-                    at(dst + 1, x) = nv[3 * x + 1];
-                    at(dst + 2, x) = nv[3 * x + 2];
-                }
         }
     };
     for (int s = 1; s <= H; s += 2) { stage(2 * R * s - 1, 0); stage(2 * R * (s + 1) - 1, 1); }

@@ -191,7 +191,8 @@ def _copy_calibration(mib):
 
 def _equations(sw, n):
     """Heat (the reference's parabola, insulated ends, Fo = 0.084) and Euler
-    (Sod, dt/dx = 0.1) at the same nX, FP64: classic, shared swept and
+    (Sod, dt/dx = 0.1) at the same nX, FP64: classic, shared swept, hybrid
+    CPU+GPU swept (the host computes the boundary node's diamond) and
     trapezoid point-updates/s (every engine gives the oracle's bits)."""
     res = {}
     x = np.arange(n) * 0.001
@@ -202,8 +203,10 @@ def _equations(sw, n):
     q[n // 2:] = (0.125, 0.0, 0.1 / 0.4)
     dt_eu = 0.1 / (n - 2)
     cases = {
-        "heat": (sw.HEAT, 1e-3, heat_ic, 20000, ((sw.CLASSIC, 256, 0), (sw.SWEPT, 1024, 0), (sw.TRAPEZOID, 256, 1))),
-        "euler": (sw.EULER, dt_eu, q.ravel(), 2000, ((sw.CLASSIC, 256, 0), (sw.SWEPT, 512, 0), (sw.TRAPEZOID, 256, 1))),
+        "heat": (sw.HEAT, 1e-3, heat_ic, 20000, ((sw.CLASSIC, 256, 0), (sw.SWEPT, 1024, 0), (sw.ALT, 1024, 0),
+                                                 (sw.TRAPEZOID, 256, 1))),
+        "euler": (sw.EULER, dt_eu, q.ravel(), 2000, ((sw.CLASSIC, 256, 0), (sw.SWEPT, 512, 0), (sw.ALT, 512, 0),
+                                                     (sw.TRAPEZOID, 256, 1))),
     }
     for name, (eq, dt, ic, steps, algs) in cases.items():
         row = {}
@@ -214,7 +217,8 @@ def _equations(sw, n):
             p.advance(k // 10)
             secs = _timed(p, k)
             p.close()
-            row[{sw.CLASSIC: "classic", sw.SWEPT: "swept", sw.TRAPEZOID: "trapezoid"}[alg]] = n * k / secs
+            row[{sw.CLASSIC: "classic", sw.SWEPT: "swept", sw.ALT: "hybrid", sw.TRAPEZOID: "trapezoid"}[alg]] = \
+                n * k / secs
         row["swept_over_classic"] = row["swept"] / row["classic"]
         res[name] = row
     res["unit"] = "point-updates/s"
