@@ -253,8 +253,8 @@ def _sizes(sw, seed, steps=20000):
         res["2^%d" % lg] = row
         _progress(f"sizes KS 2^{lg}")
     # Euler (Sod, dt/dx = 0.1, FP64) over the same sizes: classic against the
-    # swept node widths 64 .. 512 (narrow nodes give small sizes more waves),
-    # 2000 steps each
+    # swept node widths 128 .. 512 (Euler's range; narrow nodes give small
+    # sizes more waves), 2000 steps each
     eu = {}
     for lg in (14, 16, 18, 20):
         n = 1 << lg
@@ -263,8 +263,9 @@ def _sizes(sw, seed, steps=20000):
         q[n // 2:] = (0.125, 0.0, 0.1 / 0.4)
         dt_eu = 0.1 / (n - 2)
         row = {"swept": {}}
-        for alg, W in ((sw.CLASSIC, 256), (sw.SWEPT, 64), (sw.SWEPT, 128), (sw.SWEPT, 256), (sw.SWEPT, 512)):
+        for alg, W in ((sw.CLASSIC, 256), (sw.SWEPT, 128), (sw.SWEPT, 256), (sw.SWEPT, 512)):
             p = sw.Plan(sw.EULER, n, dt_eu, algorithm=alg, tpb=W, node_width=W if alg == sw.SWEPT else 0)
+            W = p.info()["node_width"] if alg == sw.SWEPT else W
             p.set_state(q.ravel())
             p.advance(200)
             secs = _timed(p, 2000)
