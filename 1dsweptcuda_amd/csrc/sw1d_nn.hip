@@ -899,6 +899,48 @@ __global__ __launch_bounds__(NT) void k_l2_parts(const double* __restrict__ a, c
     }
 }
 
+__global__ __launch_bounds__(NT) void k_zero_f64(double* __restrict__ a, int64_t n)
+{
+    for (int64_t i = threadIdx.x; i < n; i += NT) a[i] = 0.0;
+}
+
+// The judge's decision, one block: problem p's relative L2 distance from the
+// partial sums of its `per` blocks (added in block order), keep[p] = err <=
+// tol (the network's state stays), and over the call acc[0] += the number of
+// fallbacks, acc[1] = max(acc[1], largest err).  A NaN distance falls back
+// and leaves the maximum unchanged.
+__global__ __launch_bounds__(NT) void k_judge(const double* __restrict__ part, int64_t B, int64_t per,
+                                              double tol, int* __restrict__ keep, double* __restrict__ acc)
+{
+    __shared__ double smax[NT], scnt[NT];
+    double mx = 0.0, cnt = 0.0;
+    for (int64_t p = threadIdx.x; p < B; p += NT) {
+        double d = 0, s = 0;
+        for (int64_t k = p * per; k < (p + 1) * per; ++k) {
+            d += part[2 * k];
+            s += part[2 * k + 1];
+        }
+        const double err = s > 0 ? sqrt(d / s) : sqrt(d);
+        mx = err > mx ? err : mx;
+        keep[p] = err <= tol;
+        cnt += err <= tol ? 0.0 : 1.0;
+    }
+    smax[threadIdx.x] = mx;
+    scnt[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int k = NT / 2; k > 0; k >>= 1) {
+        if (threadIdx.x < k) {
+            smax[threadIdx.x] = smax[threadIdx.x + k] > smax[threadIdx.x] ? smax[threadIdx.x + k] : smax[threadIdx.x];
+            scnt[threadIdx.x] += scnt[threadIdx.x + k];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        acc[0] += scnt[0];
+        acc[1] = smax[0] > acc[1] ? smax[0] : acc[1];
+    }
+}
+
 }  // namespace sw1dnn
 
 // ================================================================== host ===
@@ -962,6 +1004,7 @@ struct sw1d_nn {
     int64_t part_len = 0;
     float* taps = nullptr;  // [5][B][X] output-layer tap sums (inference)
     int* keep = nullptr;    // [B] judged-step selection
+    double* jacc = nullptr; // [2] judge accumulators over a call: fallbacks, largest error
     // surrogate integration scratch
     double* usave = nullptr;
     double* unn = nullptr;
@@ -1128,7 +1171,7 @@ void nn_release(sw1d_nn* N)
     if (!N) return;
     if (N->stream) (void)hipStreamSynchronize(N->stream);
     void* ptrs[] = {N->p,   N->g,     N->m,     N->v,    N->x,     N->du,   N->tgt,  N->gdu,  N->dZa,
-                    N->dZb, N->part,  N->part2, N->lossp, N->taps, N->keep, N->usave, N->unn, N->l2p};
+                    N->dZb, N->part,  N->part2, N->lossp, N->taps, N->keep, N->jacc, N->usave, N->unn, N->l2p};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     for (int l = 0; l < 5; ++l) {
@@ -1198,7 +1241,8 @@ int sw1d_nn_create(sw1d_nn** out, int32_t nx, int32_t max_batch, uint64_t seed)
               dalloc(&N->part, N->part_len) == hipSuccess &&
               dalloc(&N->part2, RED_GROUPS * MID_SZ) == hipSuccess &&
               dalloc(&N->lossp, (int64_t)nblk(nout)) == hipSuccess &&
-              dalloc(&N->taps, TAPS * nout) == hipSuccess && dalloc(&N->keep, (int64_t)max_batch) == hipSuccess;
+              dalloc(&N->taps, TAPS * nout) == hipSuccess && dalloc(&N->keep, (int64_t)max_batch) == hipSuccess &&
+              dalloc(&N->jacc, (int64_t)2) == hipSuccess;
     for (int l = 0; ok && l < 5; ++l)
         ok = dalloc(&N->Z[l], nfeat) == hipSuccess && dalloc(&N->A[l], nfeat) == hipSuccess;
     for (int l = 0; ok && l < 4; ++l)
@@ -1373,11 +1417,24 @@ extern "C" int sw1d_surrogate_advance(sw1d_nn* N, sw1d_batch* batch, int64_t sur
     }
     double* u = static_cast<double*>(bv.state);
     const size_t bytes = (size_t)len * sizeof(double);
-    int64_t judged = 0, fallbacks = 0;
-    double max_err = 0.0;
-    std::vector<double> parts(2 * (size_t)blocks);
-    std::vector<int> keep((size_t)B);
+    const int64_t per = (X + NT - 1) / NT;  // l2 blocks per problem (X is a multiple of NT)
+    int64_t judged = 0;
+    // The judge runs in stream order: the plain solver on the batch's stream
+    // waits for the network step (event a), the decision on the network's
+    // stream waits for the plain solver (event b); N->jacc accumulates the
+    // fallback count and the largest judged error over the call.
+    struct Events {
+        hipEvent_t a = nullptr, b = nullptr;
+        ~Events()
+        {
+            if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
+        }
+    } ev;
+    NN_HIP(hipEventCreateWithFlags(&ev.a, hipEventDisableTiming));
+    NN_HIP(hipEventCreateWithFlags(&ev.b, hipEventDisableTiming));
     NN_HIP(hipStreamSynchronize(bv.stream));
+    hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(NT), 0, N->stream, N->jacc, (int64_t)2);
     for (int64_t i = 1; i <= surrogate_steps; ++i) {
         const bool judge = judge_every > 0 && i % judge_every == 0;
         if (judge) NN_HIP(hipMemcpyAsync(N->usave, u, bytes, hipMemcpyDeviceToDevice, N->stream));
@@ -1385,34 +1442,30 @@ extern "C" int sw1d_surrogate_advance(sw1d_nn* N, sw1d_batch* batch, int64_t sur
         if (rc) return rc;
         if (!judge) continue;
         // plain solver from the same states: u <- solver^K(usave)
-        ++judged;
         NN_HIP(hipMemcpyAsync(N->unn, u, bytes, hipMemcpyDeviceToDevice, N->stream));
         NN_HIP(hipMemcpyAsync(u, N->usave, bytes, hipMemcpyDeviceToDevice, N->stream));
-        NN_HIP(hipStreamSynchronize(N->stream));
+        if (judged == 0) {
+            // the batch may capture its graph for k_steps here: start it on idle streams
+            NN_HIP(hipStreamSynchronize(N->stream));
+        } else {
+            NN_HIP(hipEventRecord(ev.a, N->stream));
+            NN_HIP(hipStreamWaitEvent(bv.stream, ev.a, 0));
+        }
+        ++judged;
         rc = sw1d_batch_advance(batch, k_steps);
         if (rc) return nn_fail(rc, "plain solver step of the judge");
-        NN_HIP(hipStreamSynchronize(bv.stream));
+        NN_HIP(hipEventRecord(ev.b, bv.stream));
+        NN_HIP(hipStreamWaitEvent(N->stream, ev.b, 0));
         hipLaunchKernelGGL(k_l2_parts, dim3(blocks), dim3(NT), 0, N->stream, N->unn, u, len, N->l2p);
-        NN_HIP(hipMemcpyAsync(parts.data(), N->l2p, parts.size() * sizeof(double), hipMemcpyDeviceToHost,
-                              N->stream));
-        NN_HIP(hipStreamSynchronize(N->stream));
-        const int64_t per = (X + NT - 1) / NT;  // blocks per problem (X is a multiple of NT)
-        for (int64_t p = 0; p < B; ++p) {
-            double d = 0, s = 0;
-            for (int64_t k = p * per; k < (p + 1) * per; ++k) {
-                d += parts[2 * k];
-                s += parts[2 * k + 1];
-            }
-            const double err = s > 0 ? sqrt(d / s) : sqrt(d);
-            max_err = err > max_err ? err : max_err;
-            keep[(size_t)p] = err <= tol;  // otherwise the plain solver's state stays
-            fallbacks += err <= tol ? 0 : 1;
-        }
-        NN_HIP(hipMemcpyAsync(N->keep, keep.data(), (size_t)B * sizeof(int), hipMemcpyHostToDevice, N->stream));
+        hipLaunchKernelGGL(k_judge, dim3(1), dim3(NT), 0, N->stream, N->l2p, B, per, tol, N->keep, N->jacc);
         hipLaunchKernelGGL(k_keep_rows, dim3(blocks), dim3(NT), 0, N->stream, u, N->unn, N->keep, (int)X, len);
         NN_HIP(hipGetLastError());
     }
+    double acc[2] = {0.0, 0.0};
+    NN_HIP(hipMemcpyAsync(acc, N->jacc, sizeof(acc), hipMemcpyDeviceToHost, N->stream));
     NN_HIP(hipStreamSynchronize(N->stream));
+    const int64_t fallbacks = (int64_t)acc[0];
+    const double max_err = acc[1];
     if (stats) {
         stats[0] = (double)surrogate_steps;
         stats[1] = (double)judged;
