@@ -233,10 +233,11 @@ class Batch:
     """Many independent periodic heat / KS problems advanced by one persistent
     launch (sw1d_batch_*).  State is the concatenation of the fields."""
 
-    def __init__(self, equations, nx, dt, dx, *, precision: int = F64, tile_points=None):
+    def __init__(self, equations, nx, dt, dx, *, precision: int = F64, tile_points=None, waves=None):
         L = lib()
         P = C.c_void_p
         L.sw1d_batch_create.argtypes = [C.POINTER(P), C.c_int32, C.c_int64, P, P, P, P, P]
+        L.sw1d_batch_set_workgroup.argtypes = [P, C.c_int32]
         L.sw1d_batch_destroy.argtypes = [P]
         L.sw1d_batch_set_state.argtypes = [P, P]
         L.sw1d_batch_get_state.argtypes = [P, P]
@@ -255,6 +256,13 @@ class Batch:
         _check(L.sw1d_batch_create(C.byref(self._h), precision, len(self.nx), ptr(self.eq),
                                    ptr(self.nx), ptr(self.dt), ptr(self.dx), ptr(self.tp)),
                "sw1d_batch_create")
+        if waves is not None:
+            self.set_workgroup(waves)
+
+    def set_workgroup(self, waves: int) -> None:
+        """Waves per workgroup of the persistent launch (2, 4 or 8): the tiles a
+        workgroup works on at once.  Results do not depend on it."""
+        _check(lib().sw1d_batch_set_workgroup(self._h, int(waves)), "sw1d_batch_set_workgroup")
 
     def close(self):
         if self._h:

@@ -12,6 +12,11 @@ always gives the same choice: the minimum time per point-update wins, and
 candidates within `tie` of the minimum resolve to the wider tile (fewer
 redundant halo points).  Classes with no timing take the choice of the
 nearest timed class below, else above.
+
+A second pass tunes the waves per workgroup of the persistent launch (the
+tiles a workgroup works on at once, the reference's "diamonds per block"):
+the whole workload with the chosen tiles is timed at 2, 4 and 8 waves and
+`choose_waves` keeps the fastest, the default 4 when it is within `tie`.
 """
 from __future__ import annotations
 
@@ -23,6 +28,8 @@ import time
 import numpy as np
 
 CANDIDATES = (128, 256, 512, 1024)
+WAVE_CANDIDATES = (2, 4, 8)
+DEFAULT_WAVES = 4
 DEFAULT_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned", "batch_tiles.json")
 
 
@@ -41,6 +48,17 @@ def choose(timings: dict, tie: float = 0.02) -> dict:
         near = [tp for tp, v in row.items() if v <= best * (1.0 + tie)]
         out[int(cls)] = max(near)
     return out
+
+
+def choose_waves(timings: dict, tie: float = 0.02) -> int:
+    """timings: {waves: seconds per point-update} -> waves per workgroup."""
+    row = {int(k): float(v) for k, v in timings.items() if v is not None and v > 0}
+    if not row:
+        return DEFAULT_WAVES
+    best = min(row.values())
+    if DEFAULT_WAVES in row and row[DEFAULT_WAVES] <= best * (1.0 + tie):
+        return DEFAULT_WAVES
+    return min(row, key=lambda w: (row[w], w))
 
 
 def tile_points_for(nx, table: dict, default: int = 256) -> np.ndarray:
@@ -89,12 +107,32 @@ def tune_batch(sw, eq, nx, dt, dx, ics, precision: int = 0, steps: int = 400, sa
     return choose(timings), timings
 
 
-def save(table: dict, timings: dict, path: str = DEFAULT_TABLE, meta: dict | None = None) -> None:
+def tune_waves(sw, eq, nx, dt, dx, ics, table: dict, precision: int = 0, steps: int = 400, reps: int = 2,
+               candidates=WAVE_CANDIDATES):
+    """Time the whole workload, tiles from `table`, at each waves-per-workgroup
+    candidate; return (choice, {waves: seconds per point-update})."""
+    tps = tile_points_for(nx, table)
+    packed = np.concatenate(ics)
+    pts = float(np.sum(nx)) * steps
+    timings = {}
+    for w in candidates:
+        with sw.Batch(eq, nx, dt, dx, precision=precision, tile_points=tps, waves=w) as b:
+            b.set_state(packed)
+            b.advance(steps)
+            ms = min(b.advance_timed(steps) for _ in range(reps))
+        timings[w] = ms * 1e-3 / pts
+    return choose_waves(timings), timings
+
+
+def save(table: dict, timings: dict, path: str = DEFAULT_TABLE, meta: dict | None = None,
+         waves: int = DEFAULT_WAVES, wave_timings: dict | None = None) -> None:
     os.makedirs(os.path.dirname(path), exist_ok=True)
     with open(path, "w") as f:
         json.dump({"table": {str(k): v for k, v in table.items()},
                    "seconds_per_point_update": {str(k): {str(t): s for t, s in r.items()}
                                                 for k, r in timings.items()},
+                   "waves_per_workgroup": int(waves),
+                   "seconds_per_point_update_by_waves": {str(w): s for w, s in (wave_timings or {}).items()},
                    "meta": meta or {}, "written": time.strftime("%Y-%m-%d %H:%M:%S")}, f, indent=1)
 
 
@@ -105,8 +143,15 @@ def load(path: str = DEFAULT_TABLE) -> dict:
         return {int(k): int(v) for k, v in json.load(f)["table"].items()}
 
 
+def load_waves(path: str = DEFAULT_TABLE) -> int:
+    if not os.path.exists(path):
+        return DEFAULT_WAVES
+    with open(path) as f:
+        return int(json.load(f).get("waves_per_workgroup", DEFAULT_WAVES))
+
+
 def main(argv=None):
-    """Tune the batched workload's tile table on the current GPU.
+    """Tune the batched workload's tile table and waves per workgroup on the current GPU.
 
     python 1dsweptcuda_amd/autotune.py [--out PATH] [--steps N] [--precision 0|1]
     """
@@ -126,9 +171,11 @@ def main(argv=None):
     eq, nx, dt, dx = wl.batch_problems(a.count, seed=2024)
     ics = wl.batch_ics(eq, nx, dx)
     table, timings = tune_batch(sw, eq, nx, dt, dx, ics, precision=a.precision, steps=a.steps)
+    waves, wave_timings = tune_waves(sw, eq, nx, dt, dx, ics, table, precision=a.precision, steps=a.steps)
     save(table, timings, a.out, meta={"precision": a.precision, "steps": a.steps,
-                                      "workload": "batch_problems(%d, seed=2024)" % a.count})
-    print(json.dumps({"table": table}))
+                                      "workload": "batch_problems(%d, seed=2024)" % a.count},
+         waves=waves, wave_timings=wave_timings)
+    print(json.dumps({"table": table, "waves_per_workgroup": waves}))
 
 
 if __name__ == "__main__":

@@ -70,6 +70,7 @@ struct BatchLaunch {
     void* b;           // packed scratch of the same size
     int64_t nsteps;
     int32_t grid;      // persistent workgroups
+    int32_t waves;     // waves (tiles worked on at once) per workgroup: 2, 4 or 8
 };
 
 }  // namespace sw1d
@@ -88,8 +89,8 @@ bool batch_view(sw1d_batch* b, BatchView& v);
 // Launchers (sw1d_kernels.hip, sw1d_trapezoid.hip, sw1d_batch.hip).  All
 // return a hipError_t.
 hipError_t launch_batch(const BatchLaunch& L, bool f64, hipStream_t s);
-int batch_blocks_per_cu(bool f64);
-constexpr int kBatchThreads = 256;
+int batch_blocks_per_cu(bool f64, int waves);
+inline bool batch_waves_supported(int w) { return w == 2 || w == 4 || w == 8; }
 inline bool batch_tile_supported(int po) { return po == 2 || po == 4 || po == 8 || po == 16; }
 hipError_t launch_trapezoid(const TrapLaunch& L, bool f64, const Coefs& c, hipStream_t s);
 bool trapezoid_supported(int eq, int PO, int SB);

@@ -880,6 +880,8 @@ struct sw1d_batch {
     void* a = nullptr;
     void* b = nullptr;
     int32_t grid = 0;
+    int32_t waves = 4;  // waves per workgroup (tiles worked on at once)
+    int32_t cus = 0;
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -978,7 +980,8 @@ int sw1d_batch_create(sw1d_batch** out, int32_t precision, int64_t count, const 
         delete B;
         return fail(SW1D_EDEVICE, "no HIP device");
     }
-    const int64_t g = (int64_t)cus * batch_blocks_per_cu(B->f64);
+    B->cus = cus;
+    const int64_t g = (int64_t)cus * batch_blocks_per_cu(B->f64, B->waves);
     B->grid = (int32_t)std::min<int64_t>(g, count);
     const size_t bytes = (size_t)B->total * B->S;
     if (hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess ||
@@ -1035,6 +1038,7 @@ int sw1d_batch_advance(sw1d_batch* B, int64_t nsteps)
         L.b = B->b;
         L.nsteps = nsteps;
         L.grid = B->grid;
+        L.waves = B->waves;
         HIP_OR_FAIL(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
         const hipError_t le = launch_batch(L, B->f64, B->stream);
         const hipError_t ce = hipStreamEndCapture(B->stream, &B->graph);
@@ -1044,6 +1048,21 @@ int sw1d_batch_advance(sw1d_batch* B, int64_t nsteps)
         B->graph_steps = nsteps;
     }
     HIP_OR_FAIL(hipGraphLaunch(B->exec, B->stream));
+    return SW1D_OK;
+}
+
+int sw1d_batch_set_workgroup(sw1d_batch* B, int32_t waves)
+{
+    if (!B) return fail(SW1D_EINVAL, "null batch");
+    if (!batch_waves_supported(waves)) return fail(SW1D_EINVAL, "waves per workgroup must be 2, 4 or 8");
+    if (waves == B->waves) return SW1D_OK;
+    HIP_OR_FAIL(hipStreamSynchronize(B->stream));
+    if (B->exec) { (void)hipGraphExecDestroy(B->exec); B->exec = nullptr; }
+    if (B->graph) { (void)hipGraphDestroy(B->graph); B->graph = nullptr; }
+    B->graph_steps = -1;
+    B->waves = waves;
+    const int64_t g = (int64_t)B->cus * batch_blocks_per_cu(B->f64, waves);
+    B->grid = (int32_t)std::min<int64_t>(g, B->count);
     return SW1D_OK;
 }
 

@@ -318,12 +318,14 @@ def _batched(sw, count=4096, steps=5000, loop_sample=512):
     ics = wl.batch_ics(eq, nx, dx)
     table = at.load()
     tps = at.tile_points_for(nx, table) if table else None
+    waves = at.load_waves()
     total = float(nx.sum())
     res = {"problems": count, "steps": steps, "total_points": int(total),
-           "tile_table": {str(k): v for k, v in table.items()} or "size heuristic"}
+           "tile_table": {str(k): v for k, v in table.items()} or "size heuristic",
+           "waves_per_workgroup": waves}
     packed = np.concatenate(ics)
     for prec, name in ((sw.F64, "f64"), (sw.F32, "f32")):
-        with sw.Batch(eq, nx, dt, dx, precision=prec, tile_points=tps) as b:
+        with sw.Batch(eq, nx, dt, dx, precision=prec, tile_points=tps, waves=waves) as b:
             b.set_state(packed)
             b.advance(steps)  # captures the graph
             b.set_state(packed)

@@ -157,6 +157,10 @@ int sw1d_batch_set_state(sw1d_batch* batch, const void* host);
 int sw1d_batch_get_state(sw1d_batch* batch, void* host);
 int sw1d_batch_advance(sw1d_batch* batch, int64_t nsteps);
 int sw1d_batch_advance_timed(sw1d_batch* batch, int64_t nsteps, double* elapsed_ms);
+/* Waves per workgroup of the persistent launch, the tiles a workgroup works on
+ * at once (the diamonds per workgroup of the reference's tuning): 2, 4
+ * (default) or 8.  Results do not depend on it. */
+int sw1d_batch_set_workgroup(sw1d_batch* batch, int32_t waves);
 /* info[0] total points, info[1] persistent workgroups, info[2] problems,
  * info[3] device bytes */
 int sw1d_batch_info(const sw1d_batch* batch, int64_t info[4]);

@@ -63,3 +63,20 @@ def test_workload_generator_is_seeded(sw):
     assert np.allclose(8.418e-5 * dt[heat] / dx[heat] ** 2, 0.4)
     ks = ~heat
     assert np.allclose(dt[ks] * (16 / dx[ks] ** 4 - 4 / dx[ks] ** 2), 1.6)
+
+
+def test_choose_waves(at):
+    # the fastest wins; the default 4 wins a near tie; missing / bad entries are ignored
+    assert at.choose_waves({2: 3e-12, 4: 2e-12, 8: 1e-12}) == 8
+    assert at.choose_waves({2: 1.00e-12, 4: 1.01e-12, 8: 1.5e-12}) == 4
+    assert at.choose_waves({2: 1.00e-12, 4: 1.01e-12}, tie=0.0) == 2
+    assert at.choose_waves({"2": 1e-12, "8": 1e-12}) == 2  # exact tie without the default: fewer waves
+    assert at.choose_waves({}) == at.DEFAULT_WAVES == 4
+    assert at.choose_waves({2: None, 8: 0}) == 4
+
+
+def test_save_load_waves(at, tmp_path):
+    p = tmp_path / "t.json"
+    at.save({10: 128}, {10: {128: 1e-12}}, str(p), waves=8, wave_timings={2: 3e-12, 4: 2e-12, 8: 1e-12})
+    assert at.load_waves(str(p)) == 8 and at.load(str(p)) == {10: 128}
+    assert at.load_waves(str(tmp_path / "missing.json")) == 4

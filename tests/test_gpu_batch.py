@@ -13,17 +13,18 @@ def wl():
     return importlib.import_module("1dsweptcuda_amd.workloads")
 
 
+@pytest.mark.parametrize("waves", [2, 4, 8])
 @pytest.mark.parametrize("prec", [0, 1])
 @pytest.mark.parametrize("nsteps", [203, 40])
-def test_batch_matches_oracle(sw, orc, wl, prec, nsteps):
+def test_batch_matches_oracle(sw, orc, wl, prec, nsteps, waves):
     count = 40
     eq, nx, dt, dx = wl.batch_problems(count, seed=prec + 5, lo=7.0, hi=13.0)
     # every tile width, also wider than the problem (windows wrap several times)
     tps = np.array([(128, 256, 512, 1024)[i % 4] for i in range(count)], np.int32)
     dtype = np.float64 if prec == 0 else np.float32
     ics = [u.astype(dtype) for u in wl.batch_ics(eq, nx, dx)]
-    print(f"batch prec={prec} steps={nsteps} sizes={nx.min()}..{nx.max()}", flush=True)
-    with sw.Batch(eq, nx, dt, dx, precision=prec, tile_points=tps) as b:
+    print(f"batch prec={prec} steps={nsteps} waves={waves} sizes={nx.min()}..{nx.max()}", flush=True)
+    with sw.Batch(eq, nx, dt, dx, precision=prec, tile_points=tps, waves=waves) as b:
         b.set_state(np.concatenate(ics))
         b.advance(nsteps)
         got = b.split(b.get_state())
@@ -51,4 +52,29 @@ def test_batch_replay_and_repeat(sw, orc, wl):
             want = orc.heat_run(ics[i], 128, fo, fo2, 1)
         else:
             want = orc.ks_run(ics[i], 128, orc.ks_consts(dt[i], dx=dx[i]))
+        assert np.array_equal(got[i], want), i
+
+
+def test_batch_workgroup_setting(sw, orc, wl):
+    """Waves per workgroup: only 2, 4, 8; changing it between advances
+    rebuilds the launch and keeps the bits."""
+    eq, nx, dt, dx = wl.batch_problems(10, seed=9, lo=8.0, hi=12.0)
+    ics = wl.batch_ics(eq, nx, dx)
+    print("batch workgroup switch", flush=True)
+    with sw.Batch(eq, nx, dt, dx) as b:
+        with pytest.raises(sw.SweptError):
+            b.set_workgroup(3)
+        b.set_state(np.concatenate(ics))
+        b.advance(48)
+        b.set_workgroup(8)
+        b.advance(48)
+        b.set_workgroup(2)
+        b.advance(48)
+        got = b.split(b.get_state())
+    for i in range(10):
+        if eq[i] == 0:
+            fo, fo2 = orc.heat_consts(dt[i], ds=dx[i])
+            want = orc.heat_run(ics[i], 144, fo, fo2, 1)
+        else:
+            want = orc.ks_run(ics[i], 144, orc.ks_consts(dt[i], dx=dx[i]))
         assert np.array_equal(got[i], want), i
