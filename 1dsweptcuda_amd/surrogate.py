@@ -35,8 +35,11 @@ def _pkg():
 
 
 def ks_batch(sw, wl, ntraj, seed):
+    """The plain batched solver on `ntraj` seeded trajectories, with the tuned
+    waves per workgroup (the tile width of nX = 1024 is the default 512)."""
+    at = importlib.import_module("1dsweptcuda_amd.autotune")
     b = sw.Batch(np.full(ntraj, sw.KS, np.int32), np.full(ntraj, NX, np.int64), np.full(ntraj, DT),
-                 np.full(ntraj, DX))
+                 np.full(ntraj, DX), waves=at.load_waves())
     b.set_state(np.concatenate([wl.fourier_ic(NX, NX * DX, seed + i) for i in range(ntraj)]))
     return b
 
