@@ -166,3 +166,28 @@ def test_classic_graph_blocks(sw, orc, eq):
         got = p.get_state()
     want = _oracle(orc, eq, u0, sum(pieces), dt, dtype, 0, n)
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("alg", [0, 1, 3])
+def test_periodic_heat_configuration_c1(sw, orc, alg):
+    """The periodic heat case: L = 1, nX = 1024, diffusivity 8.418e-5,
+    Fo = 0.25, seeded IC 1 + sum_k a_k sin(2 pi k x), 2000 steps, node width
+    64 (tpb; trapezoid tiles of 128).  Classic, swept and trapezoid give the
+    oracle's bits, and the oracle follows the discrete decaying-mode solution,
+    each mode k damped by (1 - 4 Fo sin^2(pi k / nX)) per step."""
+    n, L, alpha = 1024, 1.0, 8.418e-5
+    dx = L / n
+    dt = 0.25 * dx * dx / alpha
+    u0, a = orc.ic_heat_fourier(n, 7, L=L)
+    fo, fo2 = orc.heat_consts(dt, ds=dx)
+    want = orc.heat_run(u0, 2000, fo, fo2, 1)
+    x = np.arange(n) * dx
+    exact = 1.0 + sum(a[k - 1] * (1.0 - 4.0 * fo * np.sin(np.pi * k / n) ** 2) ** 2000 * np.sin(2 * np.pi * k * x / L)
+                      for k in range(1, 5))
+    assert np.max(np.abs(want - exact)) < 1e-11
+    _say(f"periodic heat C1 alg={alg}")
+    with sw.Plan(0, n, dt, algorithm=alg, tpb=64, node_width=128 if alg == 3 else 0, bc=2, dx=dx) as p:
+        p.set_state(u0)
+        p.advance(2000)
+        got = p.get_state()
+    assert np.array_equal(got, want)
