@@ -242,7 +242,8 @@ def euler_exact(t, divs, thing):
     """The Sod solution on the program's cell centres dx = 1/(divs-2)
     (testProcedure.py:84-96); thing is Density, Velocity, Energy or Pressure."""
     dx = 1.0 / float(divs - 2)
-    x = np.arange(0.5 * dx, 1.0 - 0.5 * dx, dx)[: divs - 2]
+    # the divs-2 centres (j + 1/2) dx; a float-step arange can drop the last one
+    x = (np.arange(divs - 2) + 0.5) * dx
     rows, _ = sod_exact(x, t)
     return rows[["density", "velocity", "energy", "pressure"].index(thing.lower())]
 

@@ -99,3 +99,35 @@ def test_seeded_workload_options_match_oracle(tmp_path, orc, sw):
     fo, fo2 = orc.heat_consts(1e-3)
     u0, _ = orc.ic_heat_fourier(1024, 5, L=1024 * 0.001)
     assert np.array_equal(up, orc.heat_run(u0, steps, fo, fo2, 1))
+
+
+def test_sod_to_t02_at_2p14(sw):
+    """The Euler configuration at its smallest size: Sod, nX = 2^14,
+    dt/dx = 0.1, run to t = 0.2 (32764 steps; the exact solution is taken at
+    the quantised time).  Swept and classic give the same bits, and every
+    variable is closer to the exact Riemann solution than the published
+    nX = 1024 bands allow, by at least half (first-order shocks converge)."""
+    n = 1 << 14
+    dx = 1.0 / (n - 2)
+    dt = 0.1 * dx
+    steps = int(np.ceil(0.2 / dt))
+    q0 = np.zeros((n, 3))
+    q0[: n // 2] = (1.0, 0.0, 1.0 / 0.4)
+    q0[n // 2:] = (0.125, 0.0, 0.1 / 0.4)
+    out = {}
+    for alg, W in ((1, 512), (0, 256)):
+        print(f"Sod nX=2^14 alg={alg} steps={steps}", flush=True)
+        with sw.Plan(2, n, dt, algorithm=alg, tpb=W, node_width=W if alg else 0) as p:
+            p.set_state(q0.ravel())
+            p.advance(steps)
+            out[alg] = p.get_state().reshape(n, 3)
+    assert np.array_equal(out[0], out[1])
+    q = out[1][1:-1]
+    rho, m, E = q[:, 0], q[:, 1], q[:, 2]
+    u = m / rho
+    got = {"Density": rho, "Velocity": u, "Energy": E / rho - 0.5 * u * u, "Pressure": 0.4 * (E - 0.5 * m * m / rho)}
+    bands = {"Density": 8.8e-3, "Pressure": 4.5e-3, "Velocity": 1.8e-2, "Energy": 5.7e-2}
+    for name, band in bands.items():
+        err = h.rmse(h.euler_exact(steps * dt, n, name), got[name])
+        print(f"  {name} RMSE {err:.3e}", flush=True)
+        assert err < 0.5 * band, (name, err)
