@@ -13,7 +13,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "_build", "libswept1d_oracle.so")
+# SW1D_ORACLE_LIB selects another build of the same library (the sanitizer
+# build of the Makefile's `sanitize` target)
+LIB_PATH = os.environ.get("SW1D_ORACLE_LIB") or os.path.join(_HERE, "_build", "libswept1d_oracle.so")
 
 _lib = None
 
