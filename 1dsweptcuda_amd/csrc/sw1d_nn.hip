@@ -60,46 +60,26 @@ __host__ __device__ inline float bf_bits2f(uint16_t h)
 __device__ inline __bf16 f2bf(float f) { return static_cast<__bf16>(f); }
 __device__ inline float bf2f(__bf16 h) { return bf_bits2f(__builtin_bit_cast(uint16_t, h)); }
 
-// erf(x) by Abramowitz & Stegun 7.1.26 (Hastings' form): for a = |x|,
-//   erf(a) = 1 - t (a1 + t (a2 + t (a3 + t (a4 + t a5)))) exp(-a^2),  t = 1 / (1 + p a),
-// |error| <= 1.5e-7 for all a (plus FP32 rounding of the evaluation), with no
-// branches: two transcendental instructions (reciprocal, 2^x) and seven FMAs.
-__device__ inline float erf_as(float x)
-{
-    const float a = fabsf(x);
-    const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, a, 1.0f));
-    float q = fmaf(1.061405429f, t, -1.453152027f);
-    q = fmaf(q, t, 1.421413741f);
-    q = fmaf(q, t, -0.284496736f);
-    q = fmaf(q, t, 0.254829592f);
-    const float e = __builtin_amdgcn_exp2f(-1.4426950408889634f * (a * a));  // exp(-a^2)
-    return copysignf(fmaf(-q * t, e, 1.0f), x);
-}
+// GELU in its tanh form (Hendrycks & Gimpel; PyTorch's approximate="tanh"):
+//   gelu(z) = z sigmoid(c (z + a z^3)),  c = 2 sqrt(2/pi), a = 0.044715,
+// i.e. z / (1 + 2^(z (K0 + KA z^2))) with K0 = -c log2(e), KA = a K0: three
+// FP32 operations, 2^x and a reciprocal.  Large |z| give z or -0 (2^x goes to
+// 0 or inf), never NaN.  Its derivative is s + z s (1 - s) c (1 + 3a z^2),
+// s = sigmoid(c (z + a z^3)).
+constexpr float GELU_K0 = -2.3022082f;   // -c log2(e)
+constexpr float GELU_KA = -0.10294324f;  // a K0
+constexpr float GELU_C = 1.5957691f;     // c
+constexpr float GELU_C3A = 0.21406445f;  // 3 a c
 
-// erf(|z| / sqrt 2), the same approximation with the 1/sqrt 2 folded into
-// its constants (p / sqrt 2 and -log2(e) / 2).
-__device__ inline float erf_abs_scaled(float z)
+__device__ inline float gelu_sigmoid(float z)
 {
-    const float t = __builtin_amdgcn_rcpf(fmaf(0.23164189f, fabsf(z), 1.0f));
-    float q = fmaf(1.061405429f, t, -1.453152027f);
-    q = fmaf(q, t, 1.421413741f);
-    q = fmaf(q, t, -0.284496736f);
-    q = fmaf(q, t, 0.254829592f);
-    const float e = __builtin_amdgcn_exp2f((-0.72134752044448170f * z) * z);  // exp(-z^2/2)
-    return fmaf(-q * t, e, 1.0f);
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(z * fmaf(GELU_KA, z * z, GELU_K0)));
 }
-
-// GELU(z) = z/2 (1 + erf(z/sqrt 2)) = h + |h| erf(|z|/sqrt 2), h = z/2 (h and
-// z share their sign), and its derivative Phi(z) + z phi(z).
-__device__ inline float gelu(float z)
-{
-    const float h = 0.5f * z;
-    return fmaf(fabsf(h), erf_abs_scaled(z), h);
-}
+__device__ inline float gelu(float z) { return z * gelu_sigmoid(z); }
 __device__ inline float gelu_d(float z)
 {
-    const float phi = 0.39894228040143268f * __builtin_amdgcn_exp2f(-0.72134752044448170f * (z * z));
-    return fmaf(z, phi, 0.5f * (1.0f + erf_as(z * 0.70710678118654752f)));
+    const float s = gelu_sigmoid(z);
+    return fmaf(z * s * (1.0f - s), fmaf(GELU_C3A, z * z, GELU_C), s);
 }
 
 __device__ inline int wrapx(int x, int X) { return x < 0 ? x + X : (x >= X ? x - X : x); }

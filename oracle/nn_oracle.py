@@ -7,7 +7,7 @@ Same network and parameter layout as the device code:
   L1   Z1[b,x,c] = b1[c] + sum_t w1[c,t] x[b,x+t-2]             (w1: [64][5])
   L2-5 Z[b,x,co] = b[co] + sum_t sum_ci A[b,x+t-2,ci] W[t,co,ci] (W: [5][64][64])
   L6   y[b,x]    = b6 + sum_t sum_c w6[t,c] A5[b,x+t-2,c]        (w6: [5][64])
-  A = gelu(Z) (exact erf form), indices periodic in x.
+  A = gelu(Z) (tanh form), indices periodic in x.
 The backward pass is written out by hand (the reverse of each sum above).
 
 With bf16=True the oracle applies the device's precision choices: BF16
@@ -18,7 +18,6 @@ network, which the finite-difference check pins.
 from __future__ import annotations
 
 import numpy as np
-from scipy.special import erf
 
 C, TAPS = 64, 5
 SIZES = [("w1", (C, TAPS)), ("b1", (C,))]
@@ -49,12 +48,24 @@ def bf16(a):
     return (u.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
 
 
+GELU_C = 2.0 * np.sqrt(2.0 / np.pi)
+GELU_A = 0.044715
+
+
+def _gelu_sigmoid(z):
+    with np.errstate(over="ignore"):
+        return 1.0 / (1.0 + np.exp(-GELU_C * (z + GELU_A * z ** 3)))
+
+
 def gelu(z):
-    return 0.5 * z * (1.0 + erf(z / np.sqrt(2.0)))
+    """GELU in its tanh form: z sigmoid(c (z + a z^3)), c = 2 sqrt(2/pi),
+    a = 0.044715 (Hendrycks & Gimpel), the form the device evaluates."""
+    return z * _gelu_sigmoid(z)
 
 
 def gelu_d(z):
-    return 0.5 * (1.0 + erf(z / np.sqrt(2.0))) + z * np.exp(-0.5 * z * z) / np.sqrt(2.0 * np.pi)
+    s = _gelu_sigmoid(z)
+    return s + z * s * (1.0 - s) * GELU_C * (1.0 + 3.0 * GELU_A * z * z)
 
 
 def _sh(a, t):

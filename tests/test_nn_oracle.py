@@ -61,3 +61,22 @@ def test_backward_matches_finite_differences():
         pm[i] -= h
         fd = (no.loss_grad(pp, x, t, bf=False)[0] - no.loss_grad(pm, x, t, bf=False)[0]) / (2 * h)
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)) + 1e-9, (i, fd, g[i])
+
+
+def test_gelu_tanh_form():
+    """The oracle's GELU: z sigmoid(c (z + a z^3)) with c = 2 sqrt(2/pi),
+    a = 0.044715.  Pinned against its limits, the erf form it approximates
+    (within 5e-4, which a wrong constant exceeds) and, for the derivative,
+    central differences."""
+    from scipy.special import erf
+
+    z = np.linspace(-8.0, 8.0, 4001)
+    g = no.gelu(z)
+    assert no.gelu(np.array([0.0]))[0] == 0.0
+    assert np.allclose(no.gelu(np.array([30.0, -30.0])), [30.0, 0.0], atol=1e-12)
+    exact = 0.5 * z * (1.0 + erf(z / np.sqrt(2.0)))
+    assert np.max(np.abs(g - exact)) < 5e-4
+    assert np.max(np.abs(g - exact)) > 1e-5  # it is the tanh form, not erf
+    h = 1e-5
+    fd = (no.gelu(z + h) - no.gelu(z - h)) / (2 * h)
+    assert np.max(np.abs(fd - no.gelu_d(z))) < 1e-8
