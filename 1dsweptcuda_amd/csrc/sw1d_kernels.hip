@@ -358,6 +358,16 @@ struct KsRegion {
     }
 };
 
+// Ring PC of a node (1 <= PC <= P) is the 64 points that the centred region
+// of PC points per lane holds and the region of PC - 1 does not: 32 on each
+// side of the centre.  Lane l takes ring point l, so a ring moves between HBM
+// and the LDS image as two coalesced 256-byte runs per value.
+template <int W, int PC>
+__device__ __forceinline__ int ring_x(int lane)
+{
+    return lane < 32 ? W / 2 - 32 * PC + lane : W / 2 + 32 * (PC - 1) + (lane - 32);
+}
+
 // Swept KS tiles: one wave per node of W = 64*P points, lane l owning points
 // l*P .. l*P+P-1 in VGPRs.  A whole diamond runs W/2 stage levels but its
 // active set is W wide only in the middle; for P >= 4 the node's image is
@@ -366,6 +376,13 @@ struct KsRegion {
 // two-point stencil reach.  Same operations on the same values as the
 // one-width form, so the same bits, with 46% fewer point evaluations at
 // P = 16 (42% at P = 8, 31% at P = 4).
+//
+// The regions also order the node's HBM traffic.  A tile with a down phase
+// issues its loads ring by ring from the centre out and writes ring PC + 1
+// (region PC's halo) into the image just before region PC, so the outer rings
+// arrive while the inner regions compute.  A tile with an up phase stores ring
+// PC as soon as region PC has finished it, so the stores drain while the
+// inner regions compute.
 template <typename T, int P, int TILE>
 __global__ __launch_bounds__(kBlock) void k_ks_swept(const T* __restrict__ in,
                                                      T* __restrict__ out, int64_t n,
@@ -374,14 +391,30 @@ __global__ __launch_bounds__(kBlock) void k_ks_swept(const T* __restrict__ in,
 {
     constexpr int W = kWave * P, H = W / 4;
     constexpr bool PHASED = P >= 4;
+    constexpr bool RING_IN = PHASED && TILE != TILE_UP;
+    constexpr bool RING_OUT = PHASED && TILE != TILE_DOWN;
     __shared__ T img_all[PHASED ? kWavesPerBlock * 2 * W : 1];
     NodeGeom g;
     if (!node_geom<P>(n, nnodes, node_first, offset, g)) return;
 
     const T* __restrict__ in_h = in + n;
     KsRegion<T, P> full;
-    load_pts<T, P>(in, g.lb, full.u);
-    load_pts<T, P>(in_h, g.lb, full.uh);
+    T ring_u[P + 1], ring_h[P + 1];  // ring PC's point of this lane, PC = 1 .. P
+    // Node bases are multiples of W/2 and n of W, so each half of a node is
+    // contiguous in the state array: ring point x (in this lane's half) sits
+    // at hbase + x.
+    const int64_t hbase = g.lane < 32 ? g.base : wrap(g.base + W / 2, n) - W / 2;
+    if constexpr (RING_IN) {
+        for_each_index([&](auto i) {
+            constexpr int PC = decltype(i)::value + 1;
+            const int64_t gi = hbase + ring_x<W, PC>(g.lane);
+            ring_u[PC] = in[gi];
+            ring_h[PC] = in_h[gi];
+        }, std::make_integer_sequence<int, P>{});
+    } else {
+        load_pts<T, P>(in, g.lb, full.u);
+        load_pts<T, P>(in_h, g.lb, full.uh);
+    }
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
         const int64_t il = wrap(g.base - 1 - i, n), ir = wrap(g.base + W + i, n);
@@ -411,8 +444,31 @@ __global__ __launch_bounds__(kBlock) void k_ks_swept(const T* __restrict__ in,
                 full.uh[p] = img[W + lane * P + p];
             }
         };
+        // ring PC from this lane's registers into the image
+        auto put_ring = [&](auto pc_tag) {
+            constexpr int PC = decltype(pc_tag)::value;
+            const int x = ring_x<W, PC>(lane);
+            img[x] = ring_u[PC];
+            img[W + x] = ring_h[PC];
+        };
+        // ring PC from the image to HBM
+        auto store_ring = [&](auto pc_tag) {
+            constexpr int PC = decltype(pc_tag)::value;
+            const int x = ring_x<W, PC>(lane);
+            const int64_t gi = hbase + x;
+            out[gi] = img[x];
+            out[n + gi] = img[W + x];
+        };
         auto down_region = [&](auto pc_tag) {
             constexpr int PC = decltype(pc_tag)::value;
+            if constexpr (RING_IN) {
+                if constexpr (PC == 2) {
+                    put_ring(std::integral_constant<int, 1>{});
+                    put_ring(std::integral_constant<int, 2>{});
+                }
+                put_ring(std::integral_constant<int, PC + 1>{});
+                wave_sync();
+            }
             KsRegion<T, PC> r;
             r.template load<W>(img, lane);
             r.down(PC == 2 ? 1 : 16 * PC - 15, 16 * PC - 1, k);
@@ -431,11 +487,16 @@ __global__ __launch_bounds__(kBlock) void k_ks_swept(const T* __restrict__ in,
             }
             r.template save<W>(img, lane);
             wave_sync();
+            // later regions lie inside region PC - 1: ring PC is final
+            store_ring(std::integral_constant<int, PC>{});
+            if constexpr (PC == 2) store_ring(std::integral_constant<int, 1>{});
         };
         using Regions = std::make_integer_sequence<int, P - 2>;  // i -> PC = i + 2
         if (TILE != TILE_UP) {
-            full.template save<W>(img, lane);
-            wave_sync();
+            if constexpr (!RING_IN) {
+                full.template save<W>(img, lane);
+                wave_sync();
+            }
             for_each_index([&](auto i) { down_region(std::integral_constant<int, decltype(i)::value + 2>{}); },
                            Regions{});
             reload_full();
@@ -445,13 +506,15 @@ __global__ __launch_bounds__(kBlock) void k_ks_swept(const T* __restrict__ in,
             full.up(W, 1, 15, k);
             full.template save<W>(img, lane);
             wave_sync();
+            store_ring(std::integral_constant<int, P>{});
             for_each_index([&](auto i) { up_region(std::integral_constant<int, P - 1 - decltype(i)::value>{}); },
                            Regions{});
-            reload_full();
         }
     }
-    store_pts<T, P>(out, g.lb, full.u);
-    store_pts<T, P>(out + n, g.lb, full.uh);
+    if constexpr (!RING_OUT) {
+        store_pts<T, P>(out, g.lb, full.u);
+        store_pts<T, P>(out + n, g.lb, full.uh);
+    }
 }
 
 // ---------------------------------------------------------- Euler tiles ---
