@@ -178,6 +178,67 @@ struct HeatRegion {
     }
 };
 
+// Ring PC of a node (1 <= PC <= P) is the 64 points that the centred region
+// of PC points per lane holds and the region of PC - 1 does not: 32 on each
+// side of the centre.  Lane l takes ring point l, so a ring moves between HBM
+// and the LDS image as two coalesced 256-byte runs per state array.
+template <int W, int PC>
+__device__ __forceinline__ int ring_x(int lane)
+{
+    return lane < 32 ? W / 2 - 32 * PC + lane : W / 2 + 32 * (PC - 1) + (lane - 32);
+}
+
+// A node's HBM traffic ring by ring, for tiles whose levels run on centred
+// regions: NC state arrays of n values, array c at a + c n in HBM and at
+// img + c W in the node's LDS image.  A tile with a down phase issues the
+// loads of every ring from the centre out and puts ring PC + 1 (region PC's
+// halo) into the image just before region PC, so the outer rings arrive
+// while the inner regions compute.  A tile with an up phase stores ring PC as
+// soon as region PC has finished it, so the stores drain while the inner
+// regions compute.
+template <typename T, int P, int NC>
+struct NodeRings {
+    static constexpr int W = kWave * P;
+    T v[P + 1][NC];  // ring PC's point of this lane, PC = 1 .. P
+    int64_t hbase;
+    int lane;
+
+    // Node bases are multiples of W/2 and n of W, so each half of a node is
+    // contiguous in a state array: ring point x (in this lane's half) sits at
+    // hbase + x.
+    __device__ __forceinline__ NodeRings(const NodeGeom& g, int64_t n)
+        : hbase(g.lane < 32 ? g.base : wrap(g.base + W / 2, n) - W / 2), lane(g.lane)
+    {
+    }
+    template <int PC>
+    __device__ __forceinline__ void load(const T* a, int64_t n)
+    {
+        const int64_t gi = hbase + ring_x<W, PC>(lane);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) v[PC][c] = a[c * n + gi];
+    }
+    __device__ __forceinline__ void load_all(const T* a, int64_t n)
+    {
+        for_each_index([&](auto i) { load<decltype(i)::value + 1>(a, n); },
+                       std::make_integer_sequence<int, P>{});
+    }
+    template <int PC>
+    __device__ __forceinline__ void put(T* img) const
+    {
+        const int x = ring_x<W, PC>(lane);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) img[c * W + x] = v[PC][c];
+    }
+    template <int PC>
+    __device__ __forceinline__ void store(const T* img, T* a, int64_t n) const
+    {
+        const int x = ring_x<W, PC>(lane);
+        const int64_t gi = hbase + x;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) a[c * n + gi] = img[c * W + x];
+    }
+};
+
 // Orders a wave's own LDS writes before its own reads of other lanes' data:
 // LDS executes a wave's instructions in order, and the wavefront-scope
 // fences keep the compiler from moving one lane's loads above other lanes'
@@ -193,7 +254,8 @@ __device__ __forceinline__ void wave_sync()
 // node's image sits in LDS and the diamond's levels run on centred regions of
 // 1, 2, .. P points per lane, each level on the narrowest region holding its
 // active set and the one-point stencil reach:
-// the same operations, so the same bits, on fewer points.
+// the same operations, so the same bits, on fewer points.  The node moves
+// between HBM and the image ring by ring (NodeRings).
 template <typename T, int P, int TILE, bool MIRROR>
 __global__ __launch_bounds__(kBlock) void k_heat_swept(const T* __restrict__ in,
                                                        T* __restrict__ out, int64_t n,
@@ -202,14 +264,21 @@ __global__ __launch_bounds__(kBlock) void k_heat_swept(const T* __restrict__ in,
 {
     constexpr int W = kWave * P, H = W / 2;
     constexpr bool PHASED = P >= 2;
+    constexpr bool RING_IN = PHASED && TILE != TILE_UP;
+    constexpr bool RING_OUT = PHASED && TILE != TILE_DOWN;
     __shared__ T img_all[PHASED ? kWavesPerBlock * 2 * W : 1];
     NodeGeom g;
     if (!node_geom<P>(n, nnodes, node_first, offset, g)) return;
 
     const T* __restrict__ in1 = in + n;
     HeatRegion<T, P, MIRROR> full;
-    load_pts<T, P>(in, g.lb, full.r0);
-    load_pts<T, P>(in1, g.lb, full.r1);
+    NodeRings<T, P, 2> rings(g, n);
+    if constexpr (RING_IN) {
+        rings.load_all(in, n);
+    } else {
+        load_pts<T, P>(in, g.lb, full.r0);
+        load_pts<T, P>(in1, g.lb, full.r1);
+    }
     full.hl0[0] = in[wrap(g.base - 1, n)];
     full.hr0[0] = in[wrap(g.base + W, n)];
     full.hl1[0] = in1[wrap(g.base - 1, n)];
@@ -238,6 +307,11 @@ __global__ __launch_bounds__(kBlock) void k_heat_swept(const T* __restrict__ in,
         };
         auto down_region = [&](auto pc_tag) {
             constexpr int PC = decltype(pc_tag)::value;
+            if constexpr (RING_IN) {
+                if constexpr (PC == 1) rings.template put<1>(img);
+                rings.template put<PC + 1>(img);  // region PC's halo
+                wave_sync();
+            }
             HeatRegion<T, PC, MIRROR> r;
             r.template load<W>(img, lane, g.base, n);
             r.down(32 * (PC - 1) + 1, 32 * PC - 1, fo, fo2);
@@ -256,11 +330,15 @@ __global__ __launch_bounds__(kBlock) void k_heat_swept(const T* __restrict__ in,
             }
             r.template save<W>(img, lane);
             wave_sync();
+            // later regions lie inside region PC - 1: ring PC is final
+            rings.template store<PC>(img, out, n);
         };
         using Regions = std::make_integer_sequence<int, P - 1>;  // i -> PC = i + 1
         if (TILE != TILE_UP) {
-            full.template save<W>(img, lane);
-            wave_sync();
+            if constexpr (!RING_IN) {
+                full.template save<W>(img, lane);
+                wave_sync();
+            }
             for_each_index([&](auto i) { down_region(std::integral_constant<int, decltype(i)::value + 1>{}); },
                            Regions{});
             reload_full();
@@ -270,13 +348,15 @@ __global__ __launch_bounds__(kBlock) void k_heat_swept(const T* __restrict__ in,
             full.up(W, 1, 31, fo, fo2);
             full.template save<W>(img, lane);
             wave_sync();
+            rings.template store<P>(img, out, n);
             for_each_index([&](auto i) { up_region(std::integral_constant<int, P - 1 - decltype(i)::value>{}); },
                            Regions{});
-            reload_full();
         }
     }
-    store_pts<T, P>(out, g.lb, full.r0);
-    store_pts<T, P>(out + n, g.lb, full.r1);
+    if constexpr (!RING_OUT) {
+        store_pts<T, P>(out, g.lb, full.r0);
+        store_pts<T, P>(out + n, g.lb, full.r1);
+    }
 }
 
 // ------------------------------------------------------------- KS tiles ---
@@ -358,16 +438,6 @@ struct KsRegion {
     }
 };
 
-// Ring PC of a node (1 <= PC <= P) is the 64 points that the centred region
-// of PC points per lane holds and the region of PC - 1 does not: 32 on each
-// side of the centre.  Lane l takes ring point l, so a ring moves between HBM
-// and the LDS image as two coalesced 256-byte runs per value.
-template <int W, int PC>
-__device__ __forceinline__ int ring_x(int lane)
-{
-    return lane < 32 ? W / 2 - 32 * PC + lane : W / 2 + 32 * (PC - 1) + (lane - 32);
-}
-
 // Swept KS tiles: one wave per node of W = 64*P points, lane l owning points
 // l*P .. l*P+P-1 in VGPRs.  A whole diamond runs W/2 stage levels but its
 // active set is W wide only in the middle; for P >= 4 the node's image is
@@ -375,14 +445,8 @@ __device__ __forceinline__ int ring_x(int lane)
 // lane, each level on the narrowest region that holds its active set and the
 // two-point stencil reach.  Same operations on the same values as the
 // one-width form, so the same bits, with 46% fewer point evaluations at
-// P = 16 (42% at P = 8, 31% at P = 4).
-//
-// The regions also order the node's HBM traffic.  A tile with a down phase
-// issues its loads ring by ring from the centre out and writes ring PC + 1
-// (region PC's halo) into the image just before region PC, so the outer rings
-// arrive while the inner regions compute.  A tile with an up phase stores ring
-// PC as soon as region PC has finished it, so the stores drain while the
-// inner regions compute.
+// P = 16 (42% at P = 8, 31% at P = 4).  The node moves between HBM and the
+// image ring by ring (NodeRings).
 template <typename T, int P, int TILE>
 __global__ __launch_bounds__(kBlock) void k_ks_swept(const T* __restrict__ in,
                                                      T* __restrict__ out, int64_t n,
@@ -399,18 +463,9 @@ __global__ __launch_bounds__(kBlock) void k_ks_swept(const T* __restrict__ in,
 
     const T* __restrict__ in_h = in + n;
     KsRegion<T, P> full;
-    T ring_u[P + 1], ring_h[P + 1];  // ring PC's point of this lane, PC = 1 .. P
-    // Node bases are multiples of W/2 and n of W, so each half of a node is
-    // contiguous in the state array: ring point x (in this lane's half) sits
-    // at hbase + x.
-    const int64_t hbase = g.lane < 32 ? g.base : wrap(g.base + W / 2, n) - W / 2;
+    NodeRings<T, P, 2> rings(g, n);
     if constexpr (RING_IN) {
-        for_each_index([&](auto i) {
-            constexpr int PC = decltype(i)::value + 1;
-            const int64_t gi = hbase + ring_x<W, PC>(g.lane);
-            ring_u[PC] = in[gi];
-            ring_h[PC] = in_h[gi];
-        }, std::make_integer_sequence<int, P>{});
+        rings.load_all(in, n);
     } else {
         load_pts<T, P>(in, g.lb, full.u);
         load_pts<T, P>(in_h, g.lb, full.uh);
@@ -444,29 +499,14 @@ __global__ __launch_bounds__(kBlock) void k_ks_swept(const T* __restrict__ in,
                 full.uh[p] = img[W + lane * P + p];
             }
         };
-        // ring PC from this lane's registers into the image
-        auto put_ring = [&](auto pc_tag) {
-            constexpr int PC = decltype(pc_tag)::value;
-            const int x = ring_x<W, PC>(lane);
-            img[x] = ring_u[PC];
-            img[W + x] = ring_h[PC];
-        };
-        // ring PC from the image to HBM
-        auto store_ring = [&](auto pc_tag) {
-            constexpr int PC = decltype(pc_tag)::value;
-            const int x = ring_x<W, PC>(lane);
-            const int64_t gi = hbase + x;
-            out[gi] = img[x];
-            out[n + gi] = img[W + x];
-        };
         auto down_region = [&](auto pc_tag) {
             constexpr int PC = decltype(pc_tag)::value;
             if constexpr (RING_IN) {
                 if constexpr (PC == 2) {
-                    put_ring(std::integral_constant<int, 1>{});
-                    put_ring(std::integral_constant<int, 2>{});
+                    rings.template put<1>(img);
+                    rings.template put<2>(img);
                 }
-                put_ring(std::integral_constant<int, PC + 1>{});
+                rings.template put<PC + 1>(img);
                 wave_sync();
             }
             KsRegion<T, PC> r;
@@ -488,8 +528,8 @@ __global__ __launch_bounds__(kBlock) void k_ks_swept(const T* __restrict__ in,
             r.template save<W>(img, lane);
             wave_sync();
             // later regions lie inside region PC - 1: ring PC is final
-            store_ring(std::integral_constant<int, PC>{});
-            if constexpr (PC == 2) store_ring(std::integral_constant<int, 1>{});
+            rings.template store<PC>(img, out, n);
+            if constexpr (PC == 2) rings.template store<1>(img, out, n);
         };
         using Regions = std::make_integer_sequence<int, P - 2>;  // i -> PC = i + 2
         if (TILE != TILE_UP) {
@@ -506,7 +546,7 @@ __global__ __launch_bounds__(kBlock) void k_ks_swept(const T* __restrict__ in,
             full.up(W, 1, 15, k);
             full.template save<W>(img, lane);
             wave_sync();
-            store_ring(std::integral_constant<int, P>{});
+            rings.template store<P>(img, out, n);
             for_each_index([&](auto i) { up_region(std::integral_constant<int, P - 1 - decltype(i)::value>{}); },
                            Regions{});
         }
@@ -625,7 +665,10 @@ struct EulerRegion {
 // Swept Euler tiles: one wave per node of W = 64*P cells.  For P >= 4 the
 // node's image sits in LDS and the diamond's levels run on the narrowest
 // centred region of 2, 3, .. P cells per lane that holds their active set and
-// the two-cell stencil reach.  Same operations, same bits.
+// the two-cell stencil reach.  Same operations, same bits.  An Euler launch
+// is long (its stages divide and take square roots), so the node's load and
+// store are a fraction of a percent of it: the node moves whole, not by rings,
+// which keeps the ring registers out of a kernel that already uses all 256.
 template <typename T, int P, int TILE>
 __global__ __launch_bounds__(kBlock) void k_euler_swept(const T* __restrict__ in,
                                                         T* __restrict__ out, int64_t n,

@@ -40,7 +40,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per whole-diamond launch (nX, node width) -> bytes, from rocprofv3
 # FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by its reading for a copy
 # kernel of known size (profiles/ks_swept_diamond_nx1048576_hbm_bytes.md).
-DIAMOND_TRAFFIC_BYTES = {(1 << 20, 512): 33.96e6, (1 << 20, 1024): 34.75e6}
+DIAMOND_TRAFFIC_BYTES = {(1 << 20, 512): 33.96e6, (1 << 20, 1024): 34.77e6}
 
 
 _T0 = time.perf_counter()
