@@ -136,3 +136,51 @@ def test_euler_region_phases_contain_active_sets(P, pcs):
             act = x[d <= t]
             assert act.min() >= lo and act.max() < hi, (pc, t)
             assert np.all(d[halo] > t), (pc, t)
+
+
+def _ring_x(W, pc, lane):
+    """Node-local point of `lane` in ring pc (ring_x in sw1d_kernels.hip)."""
+    return W // 2 - 32 * pc + lane if lane < 32 else W // 2 + 32 * (pc - 1) + (lane - 32)
+
+
+@pytest.mark.parametrize("P,first,reach", [(4, 2, 2), (8, 2, 2), (16, 2, 2),
+                                           (2, 1, 1), (4, 1, 1), (16, 1, 1)])
+def test_rings_partition_the_regions(P, first, reach):
+    """The swept tiles move a node between HBM and its LDS image ring by ring
+    (NodeRings): the loads put ring pc + 1 into the image just before region pc
+    runs, and region pc's up-phase end stores ring pc.  That is sound when
+    rings 1..pc are exactly region pc's points (so region pc reads only rings
+    already in the image and writes no point of an outer ring), region pc's
+    stencil halo lies in ring pc + 1, and rings 1..P cover the node once.
+    KS regions start at 2 points per lane (two-point reach), heat at 1."""
+    W = 64 * P
+    seen = []
+    for pc in range(1, P + 1):
+        ring = [_ring_x(W, pc, lane) for lane in range(64)]
+        assert len(set(ring)) == 64
+        seen += ring
+        region = set(range(W // 2 - 32 * pc, W // 2 + 32 * pc))
+        assert set(seen) == region, pc
+        if first <= pc < P:
+            lo, hi = W // 2 - 32 * pc, W // 2 + 32 * pc
+            halo = set(range(lo - reach, lo)) | set(range(hi, hi + reach))
+            nxt = {_ring_x(W, pc + 1, lane) for lane in range(64)}
+            assert halo <= nxt, pc
+    assert sorted(seen) == list(range(W))
+
+
+@pytest.mark.parametrize("n,W", [(1024, 128), (4096, 1024), (1 << 20, 1024), (2048, 2048)])
+@pytest.mark.parametrize("shifted", [False, True])
+def test_ring_addresses_are_contiguous_halves(n, W, shifted):
+    """A lane's ring points sit at hbase + x in the state array (NodeRings):
+    node bases are multiples of W/2 and n of W, so neither half of a node
+    wraps, also for the tiles centred on node boundaries (offset n - W/2),
+    whose node 0 straddles the domain ends."""
+    offset = n - W // 2 if shifted else 0
+    for node in range(n // W):
+        base = (node * W + offset) % n
+        for lane in (0, 31, 32, 63):
+            hbase = base if lane < 32 else (base + W // 2) % n - W // 2
+            half = range(0, W // 2) if lane < 32 else range(W // 2, W)
+            for x in (half[0], half[-1]):
+                assert hbase + x == (base + x) % n, (node, lane, x)
