@@ -135,11 +135,20 @@ def _dtype(precision: int):
 
 
 class Plan:
-    """One solver instance on the current GPU (owns device state + a stream)."""
+    """One solver instance on the current GPU (owns device state + a stream).
+
+    node_width: 0 takes the reference's tpb; "auto" takes the autotuner's
+    width for this equation, precision and size (tuned/node_widths.json),
+    or tpb when that table has no entry."""
 
     def __init__(self, equation: int, nx: int, dt: float, *, precision: int = F64,
-                 algorithm: int = SWEPT, tpb: int = 64, node_width: int = 0,
+                 algorithm: int = SWEPT, tpb: int = 64, node_width=0,
                  bc: int = BC_DEFAULT, dx: float = 0.0, alpha: float = 0.0, tile_depth: int = 0):
+        if isinstance(node_width, str):
+            if node_width != "auto":
+                raise ValueError(f"node_width must be an int or 'auto', got {node_width!r}")
+            from . import autotune
+            node_width = autotune.node_width_for(equation, nx, precision)
         self.prob = Problem(equation, precision, algorithm, bc, nx, tpb, node_width, dt, dx, alpha,
                             tile_depth, 0)
         self.dtype = _dtype(precision)

@@ -17,6 +17,13 @@ A second pass tunes the waves per workgroup of the persistent launch (the
 tiles a workgroup works on at once, the reference's "diamonds per block"):
 the whole workload with the chosen tiles is timed at 2, 4 and 8 waves and
 `choose_waves` keeps the fastest, the default 4 when it is within `tie`.
+
+A third table serves single plans of the shared swept algorithm: the node
+width (the reference's tpb, the points one diamond spans) per equation,
+precision and size class, timed the same way and chosen by `choose`.
+`Plan(..., node_width="auto")` takes its width from that table
+(`node_width_for`); the programs keep the width the caller's tpb gives, as
+the reference's do.
 """
 from __future__ import annotations
 
@@ -31,6 +38,14 @@ CANDIDATES = (128, 256, 512, 1024)
 WAVE_CANDIDATES = (2, 4, 8)
 DEFAULT_WAVES = 4
 DEFAULT_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned", "batch_tiles.json")
+
+NODE_CANDIDATES = (128, 256, 512, 1024)
+NODE_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned", "node_widths.json")
+EQ_NAMES = ("heat", "ks", "euler")
+PREC_NAMES = ("f64", "f32")
+# timed steps per node-width candidate: a few launches of the widest node
+# (heat diamonds advance W/2 steps per launch, KS and Euler W/8)
+NODE_STEPS = {0: 4096, 1: 1024, 2: 256}
 
 
 def size_class(n: int) -> int:
@@ -150,23 +165,122 @@ def load_waves(path: str = DEFAULT_TABLE) -> int:
         return int(json.load(f).get("waves_per_workgroup", DEFAULT_WAVES))
 
 
+def _plan_case(eq: int, n: int, rng):
+    """A timing case of the equation at n points: the reference's initial
+    condition and time step (heat Fo = 0.084 on the parabola, KS dt = 0.005,
+    Sod at dt/dx = 0.1)."""
+    if eq == 0:
+        x = np.arange(n) * 0.001
+        return 100.0 * x * (1.0 - x / (0.001 * (n - 1))), 1e-3
+    if eq == 1:
+        x = np.arange(n) * 0.5
+        return 2.0 * np.cos(19.0 * np.pi * x / 128.0) + 1e-3 * rng.standard_normal(n), 0.005
+    q = np.zeros((n, 3))
+    q[: n // 2] = (1.0, 0.0, 1.0 / 0.4)
+    q[n // 2:] = (0.125, 0.0, 0.1 / 0.4)
+    return q.ravel(), 0.1 / (n - 2)
+
+
+def tune_node_widths(sw, equations=(0, 1, 2), precisions=(0, 1), log2_sizes=(12, 14, 16, 18, 20),
+                     candidates=NODE_CANDIDATES, reps: int = 2, seed: int = 0):
+    """Time the shared swept plan at every candidate node width per equation,
+    precision and size class; return (table, timings) keyed by
+    (equation, precision) then size class.  A width the plan would clamp (an
+    equation's range) or that does not split n into two nodes is skipped."""
+    rng = np.random.default_rng(seed)
+    table, timings = {}, {}
+    for eq in equations:
+        for prec in precisions:
+            rows = {}
+            for lg in log2_sizes:
+                n = 1 << lg
+                ic, dt = _plan_case(eq, n, rng)
+                steps = NODE_STEPS[eq]
+                row = {}
+                for w in candidates:
+                    if n < 2 * w:
+                        continue
+                    with sw.Plan(eq, n, dt, precision=prec, algorithm=sw.SWEPT, tpb=w, node_width=w) as p:
+                        if p.info()["node_width"] != w:
+                            continue
+                        p.set_state(ic)
+                        p.advance(steps // 4)
+                        ms = min(p.advance_timed(steps) for _ in range(reps))
+                    row[w] = ms * 1e-3 / (n * steps)
+                rows[lg] = row
+            timings[(eq, prec)] = rows
+            table[(eq, prec)] = choose(rows)
+    return table, timings
+
+
+def save_node_widths(table: dict, timings: dict, path: str = NODE_TABLE, meta: dict | None = None) -> None:
+    def nest(d, leaf):
+        out = {}
+        for (eq, prec), v in d.items():
+            out.setdefault(EQ_NAMES[eq], {})[PREC_NAMES[prec]] = leaf(v)
+        return out
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    with open(path, "w") as f:
+        json.dump({"table": nest(table, lambda r: {str(k): int(v) for k, v in r.items()}),
+                   "seconds_per_point_update": nest(timings, lambda r: {str(k): {str(w): s for w, s in c.items()}
+                                                                       for k, c in r.items()}),
+                   "meta": meta or {}, "written": time.strftime("%Y-%m-%d %H:%M:%S")}, f, indent=1)
+
+
+def load_node_widths(path: str = NODE_TABLE) -> dict:
+    """{(equation, precision): {size class: node width}}; {} without a table."""
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        t = json.load(f)["table"]
+    return {(EQ_NAMES.index(e), PREC_NAMES.index(pr)): {int(k): int(v) for k, v in r.items()}
+            for e, byp in t.items() for pr, r in byp.items()}
+
+
+def node_width_for(equation: int, n: int, precision: int = 0, table: dict | None = None) -> int:
+    """The tuned node width of a single swept plan: the entry of n's size
+    class, else of the nearest class below, else above; 0 (the caller's tpb)
+    when the table has no entry for this equation and precision."""
+    t = load_node_widths() if table is None else table
+    row = t.get((int(equation), int(precision)))
+    if not row:
+        return 0
+    c = size_class(n)
+    if c in row:
+        return row[c]
+    classes = sorted(row)
+    below = [k for k in classes if k <= c]
+    return row[below[-1]] if below else row[classes[0]]
+
+
 def main(argv=None):
-    """Tune the batched workload's tile table and waves per workgroup on the current GPU.
+    """Tune the batched workload's tile table and waves per workgroup on the
+    current GPU, or with --plans the node widths of single swept plans.
 
     python 1dsweptcuda_amd/autotune.py [--out PATH] [--steps N] [--precision 0|1]
+    python 1dsweptcuda_amd/autotune.py --plans [--out PATH]
     """
     import argparse
     import importlib
     import sys
 
     ap = argparse.ArgumentParser()
-    ap.add_argument("--out", default=DEFAULT_TABLE)
+    ap.add_argument("--out", default=None)
+    ap.add_argument("--plans", action="store_true", help="tune single-plan node widths instead")
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--precision", type=int, default=0)
     ap.add_argument("--count", type=int, default=4096)
     a = ap.parse_args(argv)
     sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
     sw = importlib.import_module("1dsweptcuda_amd")
+    if a.plans:
+        table, timings = tune_node_widths(sw)
+        save_node_widths(table, timings, a.out or NODE_TABLE,
+                         meta={"algorithm": "shared swept (1)", "log2_sizes": [12, 14, 16, 18, 20],
+                               "steps": {EQ_NAMES[k]: v for k, v in NODE_STEPS.items()}})
+        print(json.dumps({"%s/%s" % (EQ_NAMES[e], PREC_NAMES[p]): r for (e, p), r in table.items()}))
+        return
+    a.out = a.out or DEFAULT_TABLE
     wl = importlib.import_module("1dsweptcuda_amd.workloads")
     eq, nx, dt, dx = wl.batch_problems(a.count, seed=2024)
     ics = wl.batch_ics(eq, nx, dx)

@@ -80,3 +80,29 @@ def test_save_load_waves(at, tmp_path):
     at.save({10: 128}, {10: {128: 1e-12}}, str(p), waves=8, wave_timings={2: 3e-12, 4: 2e-12, 8: 1e-12})
     assert at.load_waves(str(p)) == 8 and at.load(str(p)) == {10: 128}
     assert at.load_waves(str(tmp_path / "missing.json")) == 4
+
+
+def test_node_width_lookup_and_round_trip(at, tmp_path):
+    """Single-plan node widths: exact size class, nearest class below, else
+    above; no entry -> 0 (the caller's tpb); the JSON table round-trips."""
+    table = {(1, 0): {14: 128, 18: 256, 20: 1024}, (2, 0): {16: 256}}
+    assert at.node_width_for(1, 1 << 14, 0, table) == 128
+    assert at.node_width_for(1, 1 << 19, 0, table) == 256   # class 19: nearest below is 18
+    assert at.node_width_for(1, 3 << 18, 0, table) == 256   # 786432 points: class 19
+    assert at.node_width_for(1, 3 << 19, 0, table) == 1024  # 1572864 points: class 20
+    assert at.node_width_for(1, 1 << 12, 0, table) == 128   # below every class -> lowest
+    assert at.node_width_for(2, 1 << 20, 0, table) == 256
+    assert at.node_width_for(0, 1 << 20, 0, table) == 0
+    assert at.node_width_for(1, 1 << 20, 1, table) == 0
+    timings = {k: {c: {w: 1e-12} for c, w in r.items()} for k, r in table.items()}
+    path = str(tmp_path / "nw.json")
+    at.save_node_widths(table, timings, path, meta={"test": True})
+    assert at.load_node_widths(path) == table
+    assert at.load_node_widths(str(tmp_path / "missing.json")) == {}
+
+
+def test_node_width_choice_from_timings(at):
+    """The per-class choice is `choose`: fastest, ties within 2% to the wider node."""
+    rows = {14: {128: 1.00e-12, 256: 1.015e-12, 512: 1.3e-12},
+            20: {512: 1.2e-12, 1024: 1.1e-12}}
+    assert at.choose(rows) == {14: 256, 20: 1024}

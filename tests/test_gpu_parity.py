@@ -119,6 +119,37 @@ def test_node_widths(sw, orc, eq, W):
     assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("eq,lg", [(0, 14), (1, 14), (1, 18), (2, 16)])
+def test_auto_node_width(sw, orc, eq, lg):
+    """node_width="auto" takes the autotuner's width for the equation, precision
+    and size class (tuned/node_widths.json) and gives the oracle's bits, as
+    every width does."""
+    import importlib
+    at = importlib.import_module("1dsweptcuda_amd.autotune")
+    n = 1 << lg
+    W = at.node_width_for(eq, n, 0)
+    if W == 0:
+        pytest.skip("no tuned node width for this equation")
+    dtype = np.float64
+    dt = _dt(eq, n, 0)
+    H = W // 2 if eq == 0 else W // 8
+    nsteps = 2 * H + 5
+    _say(f"auto node width eq={eq} n={n} W={W} steps={nsteps}")
+    u0 = _ic(orc, eq, n, dtype, 0)
+    with sw.Plan(eq, n, dt, algorithm=1, node_width="auto") as p:
+        assert p.info()["node_width"] == W
+        p.set_state(u0)
+        p.advance(nsteps)
+        got = p.get_state()
+    want = _oracle(orc, eq, u0, nsteps, dt, dtype, 0, n)
+    assert np.array_equal(got, want)
+
+
+def test_node_width_argument_validation(sw):
+    with pytest.raises(ValueError):
+        sw.Plan(1, 1 << 14, 0.005, node_width="widest")
+
+
 TRAP_CASES = []
 for eq, bc in ((0, 0), (0, 2), (1, 0), (2, 0)):
     for prec in (0, 1):
