@@ -145,6 +145,36 @@ def test_auto_node_width(sw, orc, eq, lg):
     assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("eq,alg,W", [(0, 1, 128), (1, 1, 128), (2, 1, 128), (1, 2, 0), (0, 3, 256),
+                                      (1, 3, 128), (2, 3, 128)])
+def test_tile_graph_blocks(sw, orc, eq, alg, W):
+    """Long advances from either buffer: trapezoid launches (alg 3) replay
+    blocks of 16 launches from a hipGraph per starting buffer, swept diamonds
+    (alg 1, and KS's register alg 2) go one by one.  Two advances of 37 tile
+    launches and a remainder each, the second starting from the other buffer,
+    give the oracle's bits, with per-kernel profiling off and on (which
+    launches one by one)."""
+    n = 1024
+    dtype = np.float64
+    dt = _dt(eq, n, 0)
+    u0 = _ic(orc, eq, n, dtype, 0)
+    results = []
+    for profiling in (False, True):
+        with sw.Plan(eq, n, dt, algorithm=alg, node_width=W, tpb=W or 64) as p:
+            per = p.info()["tile_steps"]
+            nsteps = 37 * per + 5
+            _say(f"tile graph eq={eq} alg={alg} W={p.info()['node_width']} steps=2x{nsteps} "
+                 f"profiling={profiling}")
+            p.set_profiling(profiling)
+            p.set_state(u0)
+            p.advance(nsteps)
+            p.advance(nsteps)
+            results.append(p.get_state())
+    want = _oracle(orc, eq, u0, 2 * nsteps, dt, dtype, 0, n)
+    assert np.array_equal(results[0], want)
+    assert np.array_equal(results[1], want)
+
+
 def test_node_width_argument_validation(sw):
     with pytest.raises(ValueError):
         sw.Plan(1, 1 << 14, 0.005, node_width="widest")
