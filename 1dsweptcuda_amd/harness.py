@@ -6,8 +6,9 @@ The reference drives its programs and reads their files with Python 2 helpers:
 (pyAnalysisTools/analysis_help.py:31-66), and a test procedure with exact
 solutions and a consistency run (SweptSource/Testing/testProcedure.py:51-237).
 This module keeps their names, arguments and results for the programs of this
-build (same argv, same .dat / .txt files), without plotting, and adds a reader
-for the programs' full-precision snapshots (--snapshot=FILE).
+build (same argv, same .dat / .txt files), without plotting, restates the
+timing sweep of SweptSource/runTest.sh (timing_sweep), and adds a reader for
+the programs' full-precision snapshots (--snapshot=FILE).
 
     import importlib; h = importlib.import_module("1dsweptcuda_amd.harness")
     h.runCUDA(h.program("KS"), 1024, 64, 1e-3, 20, 6, 1, "ks.dat")
@@ -132,6 +133,42 @@ def gatherTest(resultPath):
         m.index = pd.MultiIndex.from_tuples(idx, names=names)
         frames.append(m)
     return pd.concat(frames)
+
+
+# SweptSource/runTest.sh:8-38: the reference's timing sweep, 5e4 time steps of
+# each problem at its time step, output frequency 2 tf (no intermediate rows).
+SWEEP_PROBLEMS = (("Heat", 0.001), ("KS", 0.005), ("Euler", 1.0e-7))
+SWEEP_SCHEMES = ("Classic", "Shared", "Alternative")
+
+
+def timing_sweep(resultPath, nts=5e4, divisions=None, threadsPerBlock=None, precision="Double",
+                 problems=None, progress=print):
+    """The reference's timing sweep (SweptSource/runTest.sh:8-38): for every
+    problem and scheme (algorithm 0, 1, 2), every nX = 2^11 .. 2^20 and
+    tpb = 2^6 .. 2^10, `nts` time steps with output frequency 2 tf, each run
+    appending its timing line to resultPath/<Problem>_<Precision>_<Scheme>_Timing.txt
+    (after the header the reference's Python sweeps write).  gatherTest and
+    QualityRuns read the files back.  Returns the files written."""
+    os.makedirs(resultPath, exist_ok=True)
+    divisions = divisions or [2 ** k for k in range(11, 21)]
+    threadsPerBlock = threadsPerBlock or [2 ** k for k in range(6, 11)]
+    files = []
+    for prob, dt in SWEEP_PROBLEMS:
+        if problems and prob not in problems:
+            continue
+        tf = dt * nts
+        for alg, scheme in enumerate(SWEEP_SCHEMES):
+            path = op.join(resultPath, "%s_%s_%s_Timing.txt" % (prob, precision, scheme))
+            write_timing_header(path)
+            if progress:
+                progress("timing sweep: %s %s %s" % (prob, precision, scheme))
+            runCUDA(program(prob, precision), divisions, threadsPerBlock, dt, tf, 2 * tf, alg,
+                    op.join(resultPath, "temp.dat"), path, quiet=True)
+            files.append(path)
+    tmp = op.join(resultPath, "temp.dat")
+    if op.exists(tmp):
+        os.remove(tmp)
+    return files
 
 
 class QualityRuns:

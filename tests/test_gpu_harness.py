@@ -131,3 +131,21 @@ def test_sod_to_t02_at_2p14(sw):
         err = h.rmse(h.euler_exact(steps * dt, n, name), got[name])
         print(f"  {name} RMSE {err:.3e}", flush=True)
         assert err < 0.5 * band, (name, err)
+
+
+def test_timing_sweep_files(tmp_path):
+    """The timing sweep (runTest.sh:8-38) on a small grid: one file per
+    problem, precision and scheme with the reference's header, one row per
+    (nX, tpb), read back by gatherTest and QualityRuns."""
+    print("timing sweep KS, nX 2048 and 4096, tpb 64 and 128, 200 steps", flush=True)
+    files = h.timing_sweep(str(tmp_path), nts=200, divisions=[2048, 4096], threadsPerBlock=[64, 128],
+                           problems=("KS",), progress=None)
+    assert [os.path.basename(f) for f in files] == ["KS_Double_%s_Timing.txt" % s for s in h.SWEEP_SCHEMES]
+    for f in files:
+        lines = open(f).read().splitlines()
+        assert lines[0] == h.TIMING_HEADER.strip("\n") and len(lines) == 5
+    frame = h.gatherTest(str(tmp_path))
+    assert frame.shape == (6, 2) and (frame.values > 0).all()
+    best = h.QualityRuns(frame).bestRun
+    assert len(best) == 6
+    assert not os.path.exists(tmp_path / "temp.dat")
