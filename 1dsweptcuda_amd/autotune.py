@@ -39,7 +39,7 @@ WAVE_CANDIDATES = (2, 4, 8)
 DEFAULT_WAVES = 4
 DEFAULT_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned", "batch_tiles.json")
 
-NODE_CANDIDATES = (128, 256, 512, 1024)
+NODE_CANDIDATES = (64, 128, 256, 512, 1024)
 NODE_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned", "node_widths.json")
 EQ_NAMES = ("heat", "ks", "euler")
 PREC_NAMES = ("f64", "f32")

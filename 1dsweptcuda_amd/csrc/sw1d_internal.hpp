@@ -105,10 +105,11 @@ hipError_t launch_classic(int eq, int bc, bool f64, int stage, int64_t n, const 
 // every lane received its neighbour's value, 0 otherwise, <0 on HIP error.
 int check_lane_shifts();
 
-// Points per lane of a swept node (node width = 64 x that).  Radius-2
-// stencils need two; an Euler node's LDS image (six components) limits it to
-// 8 in either precision, the others to 16.
+// Points per lane of a swept node (node width = 64 x that).  One is enough
+// for every stencil (a radius-2 neighbour two lanes away arrives by a second
+// lane shift); an Euler node's LDS image (six components) limits it to 8 in
+// either precision, the others to 16.
 inline int max_points_per_lane(int eq, bool /*f64*/) { return eq == EQ_EULER ? 8 : 16; }
-inline int min_points_per_lane(int eq) { return eq == EQ_HEAT ? 1 : 2; }
+inline int min_points_per_lane(int) { return 1; }
 
 }  // namespace sw1d

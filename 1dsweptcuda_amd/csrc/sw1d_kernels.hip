@@ -571,6 +571,9 @@ struct EulerRegion {
     T hl[6][2], hr[6][2];
     int d[PC];
     bool g0, gN;  // this lane's first / last cell is global cell 0 / n-1
+    // one cell per lane: the lane's cell is global cell 1 / n-2 (with two or
+    // more per lane, cells 0 and 1, and n-2 and n-1, share a lane)
+    bool g1 = false, gNm1 = false;
 
     // src = 0: predictor from the conserved values (Euler:307-350);
     // src = 3: corrector from the predictor values (Euler:353-389, :420).
@@ -593,8 +596,13 @@ struct EulerRegion {
         // of global cell 1 drops pdiff(-1,0), the one right of cell n-2 drops
         // pdiff(n-1,n) (the flags of Euler:405, :424).
         auto iface = [&](int j, Q3<T>& F, Q3<T>& D) {
-            const T q0 = (j == 1 && g0) ? T(0) : pd[j];
-            const T q2 = (j == PC - 1 && gN) ? T(0) : pd[j + 2];
+            bool drop0 = j == 1 && g0, drop2 = j == PC - 1 && gN;
+            if constexpr (PC == 1) {
+                drop0 = drop0 || (j == 0 && g1);
+                drop2 = drop2 || (j == 1 && gNm1);
+            }
+            const T q0 = drop0 ? T(0) : pd[j];
+            const T q2 = drop2 ? T(0) : pd[j + 2];
             eu_iface(cell(j), cell(j + 1), cell(j + 2), cell(j + 3), q0, pd[j + 1], q2, k, F, D);
         };
         const T c = src == 0 ? T(0.25) * k.dtdx : T(0.5) * k.dtdx;
@@ -636,6 +644,10 @@ struct EulerRegion {
         gl = gl >= n ? gl - n : gl;
         g0 = gi == 0;
         gN = gl == n - 1;
+        if (PC == 1) {
+            g1 = gi == 1;
+            gNm1 = gl == n - 2;
+        }
     }
     template <int W>
     __device__ __forceinline__ void load(const T* img, int lane, int64_t base, int64_t n)
@@ -909,13 +921,10 @@ hipError_t swept_tile(const SweptLaunch& L, const Coefs& c, hipStream_t s)
             hipLaunchKernelGGL((k_heat_swept<T, P, TILE, true>), grid, dim3(kBlock), 0, s, in, out,
                                L.n, L.nnodes, L.node_first, L.offset, fo_of<T>(c), fo2_of<T>(c));
     } else if (L.eq == EQ_KS) {
-        if constexpr (P >= 2)
-            hipLaunchKernelGGL((k_ks_swept<T, P, TILE>), grid, dim3(kBlock), 0, s, in, out, L.n,
-                               L.nnodes, L.node_first, L.offset, ks_of<T>(c));
-        else
-            return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_ks_swept<T, P, TILE>), grid, dim3(kBlock), 0, s, in, out, L.n,
+                           L.nnodes, L.node_first, L.offset, ks_of<T>(c));
     } else {
-        if constexpr (P >= 2 && P <= 8)
+        if constexpr (P <= 8)
             hipLaunchKernelGGL((k_euler_swept<T, P, TILE>), grid, dim3(kBlock), 0, s, in, out,
                                L.n, L.nnodes, L.node_first, L.offset, eu_of<T>(c));
         else

@@ -41,17 +41,19 @@ __device__ __forceinline__ float from_right(float v, float old)
 }
 
 // e[0 .. P+2R-1] = values at node-local x = l*P - R .. l*P + P + R - 1.
+// The point at x = l*P - 1 - i is lane l-1's e[R - 1 - i + P]: its own point
+// P-1-i when i < P, else (a radius wider than a lane, R > P) the value lane
+// l-1 received i - P shifts earlier.  Likewise on the right.
 template <typename T, int P, int R>
 __device__ __forceinline__ void gather(const T (&v)[P], const T (&hl)[R], const T (&hr)[R],
                                        T (&e)[P + 2 * R])
 {
-    static_assert(P >= R, "a lane must hold at least one stencil radius");
 #pragma unroll
     for (int p = 0; p < P; ++p) e[R + p] = v[p];
 #pragma unroll
     for (int i = 0; i < R; ++i) {
-        e[R - 1 - i] = from_left(v[P - 1 - i], hl[i]);
-        e[R + P + i] = from_right(v[i], hr[i]);
+        e[R - 1 - i] = from_left(e[R - 1 - i + P], hl[i]);
+        e[R + P + i] = from_right(e[R + i], hr[i]);
     }
 }
 
@@ -64,13 +66,12 @@ template <typename T, int P, int R>
 __device__ __forceinline__ void gather_carry(const T (&v)[P], T (&nl)[R], T (&nr)[R],
                                              T (&e)[P + 2 * R])
 {
-    static_assert(P >= R, "a lane must hold at least one stencil radius");
 #pragma unroll
     for (int p = 0; p < P; ++p) e[R + p] = v[p];
 #pragma unroll
     for (int i = 0; i < R; ++i) {
-        nl[i] = from_left(v[P - 1 - i], nl[i]);
-        nr[i] = from_right(v[i], nr[i]);
+        nl[i] = from_left(e[R - 1 - i + P], nl[i]);
+        nr[i] = from_right(e[R + i], nr[i]);
         e[R - 1 - i] = nl[i];
         e[R + P + i] = nr[i];
     }

@@ -235,7 +235,7 @@ def _equations(sw, n):
 
 def _sizes(sw, seed, steps=20000):
     """The KS grid of the reference's performance sweeps, FP64: nX = 2^14 ..
-    2^20 with swept node widths 128 .. 1024 (the reference's tpb), the
+    2^20 with swept node widths 64 .. 1024 (the reference's tpb), the
     classic algorithm and the register variant (alg 2), 20000 steps each;
     Euler (Sod) classic and swept over the same sizes; and KS FP32 at 2^20."""
     res = {}
@@ -244,7 +244,7 @@ def _sizes(sw, seed, steps=20000):
         u0 = _ks_ic(n, seed)
         row = {"swept": {}}
         for alg, name, widths in ((sw.CLASSIC, "classic", (256,)), (sw.ALT, "register", (128,)),
-                                  (sw.SWEPT, "swept", (128, 256, 512, 1024))):
+                                  (sw.SWEPT, "swept", (64, 128, 256, 512, 1024))):
             for W in widths:
                 p = sw.Plan(sw.KS, n, 0.005, algorithm=alg, tpb=W, node_width=W if alg == sw.SWEPT else 0)
                 p.set_state(u0)
@@ -261,8 +261,8 @@ def _sizes(sw, seed, steps=20000):
         res["2^%d" % lg] = row
         _progress(f"sizes KS 2^{lg}")
     # Euler (Sod, dt/dx = 0.1, FP64) over the same sizes: classic against the
-    # swept node widths 128 .. 512 (Euler's range; narrow nodes give small
-    # sizes more waves), 2000 steps each
+    # swept node widths 64 .. 512 (Euler's range; narrow nodes give small
+    # sizes more waves and less serial work per wave), 2000 steps each
     eu = {}
     for lg in (14, 16, 18, 20):
         n = 1 << lg
@@ -271,7 +271,7 @@ def _sizes(sw, seed, steps=20000):
         q[n // 2:] = (0.125, 0.0, 0.1 / 0.4)
         dt_eu = 0.1 / (n - 2)
         row = {"swept": {}}
-        for alg, W in ((sw.CLASSIC, 256), (sw.SWEPT, 128), (sw.SWEPT, 256), (sw.SWEPT, 512)):
+        for alg, W in ((sw.CLASSIC, 256), (sw.SWEPT, 64), (sw.SWEPT, 128), (sw.SWEPT, 256), (sw.SWEPT, 512)):
             p = sw.Plan(sw.EULER, n, dt_eu, algorithm=alg, tpb=W, node_width=W if alg == sw.SWEPT else 0)
             W = p.info()["node_width"] if alg == sw.SWEPT else W
             p.set_state(q.ravel())

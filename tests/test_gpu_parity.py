@@ -99,8 +99,8 @@ def test_ks_phased_diamond_single_precision(sw, orc, W):
 @pytest.mark.parametrize("eq", [0, 1, 2])
 @pytest.mark.parametrize("W", [64, 128, 256, 512, 1024])
 def test_node_widths(sw, orc, eq, W):
-    if eq != 0 and W < 128:
-        pytest.skip("radius-2 stages need >= 2 points per lane")
+    """Every node width the equation allows, 64 included: one point per lane,
+    where a radius-2 stencil's outer neighbour comes from two lanes away."""
     if eq == 2 and W > 512:
         pytest.skip("Euler nodes stop at 512 points")
     n = 8 * W

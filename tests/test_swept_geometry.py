@@ -29,7 +29,7 @@ def test_heat_swept_equals_classic(orc, dtype, W, nsteps, bc):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-@pytest.mark.parametrize("W,nsteps", [(128, 16 * 5), (256, 32 * 3 + 5), (512, 64)])
+@pytest.mark.parametrize("W,nsteps", [(64, 8 * 5 + 3), (128, 16 * 5), (256, 32 * 3 + 5), (512, 64)])
 def test_ks_swept_equals_classic(orc, dtype, W, nsteps):
     n = 2048
     e = Emu("ks", n, 0.005, dtype, W=W)
@@ -40,7 +40,7 @@ def test_ks_swept_equals_classic(orc, dtype, W, nsteps):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-@pytest.mark.parametrize("W,nsteps", [(128, 16 * 7), (256, 32 * 2 + 3)])
+@pytest.mark.parametrize("W,nsteps", [(64, 8 * 5 + 1), (128, 16 * 7), (256, 32 * 2 + 3)])
 def test_euler_swept_equals_classic(orc, dtype, W, nsteps):
     n = 1024
     dx = 1.0 / (n - 2)
