@@ -97,6 +97,8 @@ def lib():
         L.sw1d_schedule.argtypes = [C.c_int32, C.c_double, C.c_double, C.c_double, C.c_int32,
                                     C.c_double, C.POINTER(C.c_int64), C.POINTER(C.c_double),
                                     C.c_int64, C.POINTER(C.c_int64)]
+        L.sw1d_initial_condition.argtypes = [C.c_int32, C.c_int32, C.c_int64, C.c_double, C.c_double,
+                                             C.c_uint64, P]
         _lib = L
     return _lib
 
@@ -128,6 +130,29 @@ def schedule(kind: int, dt: float, tf_arg: float, freq: float, tpb: int,
     _check(L.sw1d_schedule(kind, dt, tf_arg, freq, tpb, dt_sched, steps, times, count.value,
                            C.byref(count)), "sw1d_schedule")
     return list(steps), list(times)
+
+
+IC_KS_NOISY, IC_HEAT_FOURIER = 0, 1
+
+
+def initial_condition(kind: int, n: int, length: float, param: float, seed: int,
+                      precision: int = F64) -> np.ndarray:
+    """Seeded synthetic initial condition drawn by the library (splitmix64 on
+    the host; sw1d_initial_condition)."""
+    out = np.empty(n, _dtype(precision))
+    _check(lib().sw1d_initial_condition(kind, precision, n, length, param, seed,
+                                        out.ctypes.data_as(C.c_void_p)), "sw1d_initial_condition")
+    return out
+
+
+def ks_noisy_ic(n: int, seed: int, dx: float = 0.5, sigma: float = 1e-3, precision: int = F64):
+    """2cos(19 pi x/128) + sigma N(0,1) at x = k dx (the KS benchmark workload)."""
+    return initial_condition(IC_KS_NOISY, n, dx, sigma, seed, precision)
+
+
+def heat_fourier_ic(n: int, seed: int, L: float = 1.0, precision: int = F64):
+    """1 + sum_{k=1..4} a_k sin(2 pi k x/L), a_k ~ U[0,1) (the periodic heat workload)."""
+    return initial_condition(IC_HEAT_FOURIER, n, L, 0.0, seed, precision)
 
 
 def _dtype(precision: int):

@@ -139,22 +139,6 @@ Options take_options(int& argc, char* argv[])
     return o;
 }
 
-// splitmix64 (Steele, Lea, Flood 2014) and the draws made from it.
-uint64_t splitmix64(uint64_t& s)
-{
-    uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
-    return z ^ (z >> 31);
-}
-double uniform01(uint64_t& s) { return (double)(splitmix64(s) >> 11) * (1.0 / 9007199254740992.0); }
-[[maybe_unused]] double normal01(uint64_t& s)  // Box-Muller, cosine branch
-{
-    const double u1 = 1.0 - uniform01(s);
-    const double u2 = uniform01(s);
-    return sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
-}
-
 const char kSnapMagic[8] = {'S', 'W', '1', 'D', 'S', 'N', 'P', '1'};
 constexpr int kSnapPrecision = (sizeof(REAL) == 8) ? 0 : 1;
 
@@ -287,17 +271,10 @@ int main(int argc, char* argv[])
     IC.resize((size_t)dv);
     const REAL lxr = (REAL)lx;
     if (opt.ic == "fourier") {
-        // periodic workload: 1 + sum_{k=1..4} a_k sin(2 pi k x / L), x = j L / nX
-        uint64_t st = opt.seed;
-        double amp[4];
-        for (double& a : amp) a = uniform01(st);
+        // periodic workload: 1 + sum_{k=1..4} a_k sin(2 pi k x/L), a_k ~ U[0,1), L = nX ds
         const double L = (double)dv * (double)ds;
-        for (int j = 0; j < dv; j++) {
-            const double x = (double)j * (L / (double)dv);
-            double v = 1.0;
-            for (int k = 0; k < 4; k++) v += amp[k] * sin(2.0 * M_PI * (double)(k + 1) * x / L);
-            IC[j] = (REAL)v;
-        }
+        if (sw1d_initial_condition(SW1D_IC_HEAT_FOURIER, kSnapPrecision, dv, L, 0.0, opt.seed, IC.data()))
+            die_lib("initial condition");
     } else if (opt.ic == "reference") {
         for (int k = 0; k < dv; k++) {
             const REAL a = ((REAL)k * ds);
@@ -321,11 +298,8 @@ int main(int argc, char* argv[])
     IC.resize((size_t)dv);
     if (opt.ic == "noisy") {
         // KS workload: reference IC plus N(0, 1e-3) noise per point, x = k dx
-        uint64_t st = opt.seed;
-        for (int k = 0; k < dv; k++) {
-            const double x = (double)k * (double)dx;
-            IC[k] = (REAL)(2.0 * cos(19.0 * x * M_PI / 128.0) + 1e-3 * normal01(st));
-        }
+        if (sw1d_initial_condition(SW1D_IC_KS_NOISY, kSnapPrecision, dv, (double)dx, 1e-3, opt.seed, IC.data()))
+            die_lib("initial condition");
     } else if (opt.ic == "reference") {
         for (int k = 0; k < dv; k++) {
             const REAL x = (REAL)k * (REAL)dx;

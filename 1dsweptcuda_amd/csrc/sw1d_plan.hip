@@ -175,6 +175,26 @@ void fill_coefs(sw1d_plan* p, double* euler_dx)
 
 bool is_pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
 
+// splitmix64 (Steele, Lea, Flood 2014): a portable generator, so the same seed
+// gives the same synthetic initial condition on every host.
+uint64_t splitmix64(uint64_t& s)
+{
+    uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+double splitmix_uniform(uint64_t& s)  // [0, 1), 53 random bits
+{
+    return (double)(splitmix64(s) >> 11) * (1.0 / 9007199254740992.0);
+}
+double splitmix_normal(uint64_t& s)  // Box-Muller, cosine branch
+{
+    const double u1 = 1.0 - splitmix_uniform(s);
+    const double u2 = splitmix_uniform(s);
+    return sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
+}
+
 // ------------------------------------------------------------ profiling ---
 hipEvent_t take_event(sw1d_plan* p)
 {
@@ -924,6 +944,45 @@ int sw1d_schedule(int32_t kind, double dt, double tf_arg, double freq, int32_t t
     for (size_t i = 0; i < rows.size(); ++i) {
         steps[i] = rows[i].steps;
         times[i] = rows[i].time;
+    }
+    return SW1D_OK;
+}
+
+int sw1d_initial_condition(int32_t kind, int32_t precision, int64_t n, double length, double param,
+                           uint64_t seed, void* host)
+{
+    if (!host) return fail(SW1D_EINVAL, "null output buffer");
+    if (n < 1) return fail(SW1D_EINVAL, "n must be positive, got %lld", (long long)n);
+    if (precision != SW1D_F64 && precision != SW1D_F32)
+        return fail(SW1D_EINVAL, "unknown precision %d", precision);
+    if (!(length > 0.0)) return fail(SW1D_EINVAL, "length/spacing must be positive");
+    std::vector<double> u((size_t)n);
+    uint64_t st = seed;
+    if (kind == SW1D_IC_KS_NOISY) {
+        // 2cos(19 pi x/128) at x = k dx (KS1D_SweptShared.cu:68-72, 945), plus
+        // param * N(0,1) per point
+        for (int64_t k = 0; k < n; ++k) {
+            const double x = (double)k * length;
+            u[(size_t)k] = 2.0 * cos(19.0 * x * M_PI / 128.0) + param * splitmix_normal(st);
+        }
+    } else if (kind == SW1D_IC_HEAT_FOURIER) {
+        // 1 + sum_{k=1..4} a_k sin(2 pi k x / L), a_k ~ U[0,1), x = j L / n
+        double a[4];
+        for (double& v : a) v = splitmix_uniform(st);
+        for (int64_t j = 0; j < n; ++j) {
+            const double x = (double)j * (length / (double)n);
+            double v = 1.0;
+            for (int k = 0; k < 4; ++k) v += a[k] * sin(2.0 * M_PI * (double)(k + 1) * x / length);
+            u[(size_t)j] = v;
+        }
+    } else {
+        return fail(SW1D_EINVAL, "unknown initial condition kind %d", kind);
+    }
+    if (precision == SW1D_F64) {
+        std::memcpy(host, u.data(), sizeof(double) * (size_t)n);
+    } else {
+        float* f = static_cast<float*>(host);
+        for (int64_t k = 0; k < n; ++k) f[k] = (float)u[(size_t)k];
     }
     return SW1D_OK;
 }

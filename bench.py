@@ -2,10 +2,13 @@
 
 Workload (default): Kuramoto-Sivashinsky, FP64, nX = 2^20, dx = 0.5,
 dt = 0.005, IC 2cos(19 pi x/128) + N(0, 1e-3) noise (seed 1), shared swept
-algorithm.  A "step" is one time step of the whole domain (nX point-updates);
-the timed region is exactly --steps steps with the state already in HBM,
-bracketed by device synchronisation.  The classic algorithm (one launch per RK
-stage) is timed on the same input for the swept/classic ratio.
+algorithm.  A "step" is one time step of the whole domain (nX point-updates).
+The timed region is --steps steps rounded up to whole swept tile launches (a
+diamond advances W/8 = 128 steps at node width 1024; at least 160 launches),
+with the state already in HBM, bracketed by device synchronisation; "steps"
+in the output is the count actually timed.  Warm-up is rounded the same way.
+The classic algorithm (one launch per RK stage) is timed on the same input
+for the swept/classic ratio.
 
 The roofline entry is for the dominant kernel (the swept diamond): its
 average duration comes from hipEvents recorded around every launch on the
@@ -15,7 +18,9 @@ duration.
 
 The CPU baseline is the C oracle (the same KS stages) on a bounded sample of
 the same workload (same nX, fewer steps), timed on this host with OpenMP over
-the threads this job may use (at most 16) and with one thread.
+the threads this job may use (at most 16) and with one thread.  The same leg
+states the headline's error: the swept plan and the oracle advance the same
+initial condition by the same whole number of tile launches and are compared.
 
 Prints one JSON line.
 """
@@ -41,6 +46,16 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by its reading for a copy
 # kernel of known size (profiles/ks_swept_diamond_nx1048576_hbm_bytes.md).
 DIAMOND_TRAFFIC_BYTES = {(1 << 20, 512): 33.96e6, (1 << 20, 1024): 34.77e6}
+METRIC = "point-updates/s on 1 GPU at stated error vs FP64 oracle: swept vs classic/batch/NN"
+# The reference's published figure for the same quantity: KS FP64 shared swept at
+# nX = 2^20, 105.408 us per time step on a Tesla K40c
+# (ResultPlots/ConferencePaper/BestRuns.csv:14).
+REFERENCE_KS_SWEPT_PU_PER_S = (1 << 20) / 105.408e-6
+# Timed and warm-up steps are whole tile launches (a swept advance of fewer
+# steps than one tile runs classic steps instead), at least this many of them:
+# 160 launches of the 2^20-point diamond are ~25 ms of GPU time.
+MIN_TIMED_LAUNCHES = 160
+MIN_WARM_LAUNCHES = 160
 
 
 _T0 = time.perf_counter()
@@ -51,12 +66,20 @@ def _progress(section):
     print(f"[bench] {section} done at {time.perf_counter() - _T0:.1f} s", file=sys.stderr, flush=True)
 
 
-def _warm(plan, steps):
-    """Untimed steps before a timed run: at least `steps`, and at least 40
-    tile launches, so the plan's one-time set-up (the launch graphs it
-    captures on its first long advance) stays out of the timed region."""
-    per = plan.info()["tile_steps"]
-    plan.advance(max(steps, 40 * per) if per > 0 else steps)
+def _whole_launches(per, requested, minimum):
+    """`requested` steps rounded up to whole tile launches of `per` steps, and
+    at least `minimum` launches (per <= 1: the plan has no tiles)."""
+    if per <= 1:
+        return max(requested, minimum)
+    return max(minimum, -(-requested // per)) * per
+
+
+def _warm(plan, steps, minimum=40):
+    """Untimed steps before a timed run: at least `steps`, rounded up to whole
+    tile launches, and at least `minimum` launches, so the plan's one-time
+    set-up (the launch graphs it captures on its first long advance) stays out
+    of the timed region."""
+    plan.advance(_whole_launches(plan.info()["tile_steps"], steps, minimum))
 
 
 def _timed(plan, steps):
@@ -70,11 +93,10 @@ def _timed(plan, steps):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20000)
-    # 20000 untimed steps (~26 ms at 2^20): with 2000, the first timed steps of a
-    # fresh process still run with gaps between launches (same kernel times,
-    # ~5% lower wall-clock rate)
-    ap.add_argument("--warmup", type=int, default=20000)
+    ap.add_argument("--steps", type=int, default=20000,
+                    help="timed time steps; rounded up to whole tile launches, at least %d" % MIN_TIMED_LAUNCHES)
+    ap.add_argument("--warmup", type=int, default=20000,
+                    help="untimed time steps; rounded up the same way, at least %d launches" % MIN_WARM_LAUNCHES)
     ap.add_argument("--nx", type=int, default=1 << 20)
     ap.add_argument("--node-width", type=int, default=1024)
     ap.add_argument("--seed", type=int, default=1)
@@ -99,37 +121,45 @@ def main():
 
     sw = importlib.import_module("1dsweptcuda_amd")
     n, dt = args.nx, 0.005
-    rng_ic = _ks_ic(n, args.seed)
+    u0 = sw.ks_noisy_ic(n, args.seed)  # the field tests/test_gpu_fullsize.py checks bit for bit
 
     # ---------------------------------------------------------- swept (alg 1)
     plan = sw.Plan(sw.KS, n, dt, algorithm=sw.SWEPT, tpb=args.node_width, node_width=args.node_width)
     info = plan.info()
-    plan.set_state(rng_ic)
-    plan.advance(args.warmup)
-    secs = _timed(plan, args.steps)
-    value = n * args.steps / secs
+    H, W = info["tile_steps"], info["node_width"]
+    if H <= 1:
+        raise SystemExit(f"bench: nX={n} with node width {args.node_width} gives no swept tiles")
+    steps = _whole_launches(H, args.steps, MIN_TIMED_LAUNCHES)
+    warm = _whole_launches(H, args.warmup, MIN_WARM_LAUNCHES)
+    plan.set_state(u0)
+    plan.advance(warm)
+    secs = _timed(plan, steps)
+    value = n * steps / secs
 
-    # per-kernel durations over the same number of steps
+    # per-kernel durations over the same number of steps: hipEvents recorded
+    # around every launch on the plan's stream
     plan.set_profiling(True)
-    plan.advance(args.steps)
+    plan.advance(steps)
     plan.synchronize()
     dn, dms = plan.kernel_stats(sw.K_DIAMOND)
     sn, sms = plan.kernel_stats(sw.K_SPLIT)
     plan.set_profiling(False)
+    plan.close()
     launches = dn + sn
-    avg_ms = (dms + sms) / max(launches, 1)
-    W = info["node_width"]
-    pu_per_launch = n * (W // 8)
+    if launches == 0 or dms + sms <= 0.0:
+        raise SystemExit(f"bench: the profiled pass of {steps} steps recorded no diamond launch "
+                         f"(tile height {H} steps)")
+    avg_ms = (dms + sms) / launches
+    pu_per_launch = n * H
     flops = KS_FLOP_PER_PU * pu_per_launch
     achieved_tf = flops / (avg_ms * 1e-3) / 1e12
-    plan.close()
     _progress("swept")
 
     # ------------------------------------------------------- classic (alg 0)
     cplan = sw.Plan(sw.KS, n, dt, algorithm=sw.CLASSIC, tpb=256)
-    cplan.set_state(rng_ic)
-    cplan.advance(min(args.warmup, 500))
-    csteps = max(args.steps // 4, 1)
+    cplan.set_state(u0)
+    cplan.advance(512)  # includes the capture of the 64-step launch graph
+    csteps = max(steps // 4, 4096)
     csecs = _timed(cplan, csteps)
     classic_value = n * csteps / csecs
     cplan.close()
@@ -140,37 +170,45 @@ def main():
     _progress("classic")
 
     out = {
-        "metric": "point-updates/s (KS FP64 swept, nX=2^20, bitwise equal to the FP64 CPU oracle)",
+        "metric": METRIC,
         "value": value,
         "unit": "point-updates/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": secs / args.steps * 1e3,
+        "steps": steps,
+        "warmup": warm,
+        "steps_requested": args.steps,
+        "warmup_requested": args.warmup,
+        "ms_per_step": secs / steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
-        "vs_baseline": None,
+        "vs_baseline": value / REFERENCE_KS_SWEPT_PU_PER_S,
+        "baseline": {"value": REFERENCE_KS_SWEPT_PU_PER_S, "unit": "point-updates/s",
+                     "source": "reference KS FP64 shared swept, nX=2^20, 105.408 us/step on a Tesla K40c "
+                               "(ResultPlots/ConferencePaper/BestRuns.csv:14)"},
         "dtype": "f64",
-        "data": "synthetic (seeded IC 2cos(19 pi x/128) + N(0,1e-3), seed %d)" % args.seed,
-        "config": {"workload": "KS 1D, RK2 + central differences, FP64, periodic",
+        "data": "synthetic: IC 2cos(19 pi x/128) + N(0,1e-3) drawn by sw1d_initial_condition "
+                "(splitmix64, seed %d)" % args.seed,
+        "config": {"workload": "Kuramoto-Sivashinsky 1D, RK2 + central differences, FP64, periodic, "
+                               "shared swept (alg 1)",
                    "nx": n, "dx": 0.5, "dt": dt, "algorithm": "shared swept (alg 1)",
-                   "node_width": W, "steps_per_tile_launch": W // 8},
-        "classic": {"value": classic_value, "ms_per_step": csecs / csteps * 1e3,
+                   "node_width": W, "steps_per_tile_launch": H},
+        "classic": {"value": classic_value, "steps": csteps, "ms_per_step": csecs / csteps * 1e3,
                     "roofline": {"bound": "hbm", "achieved": classic_gbs, "peak": HBM_PEAK_GBS,
                                  "unit": "GB/s", "frac": classic_gbs / HBM_PEAK_GBS,
                                  "algorithmic_bytes_per_pu": 40}},
         "swept_over_classic": value / classic_value,
-        "roofline": {"bound": "mfma", "compute": "fp64 vector ALU (no MFMA); peak = MI355X FP64",
+        "roofline": {"bound": "fp64_valu", "compute": "FP64 vector ALU (no MFMA); peak = MI355X FP64 vector",
                      "kernel": "k_ks_swept<double,%d,DIAMOND>" % (W // 64),
                      "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved_tf / FP64_PEAK_TFLOPS,
                      "traffic": DIAMOND_TRAFFIC_BYTES.get((n, W)),
                      "avg_launch_us": avg_ms * 1e3, "launches": launches,
+                     "wall_us_per_tile_launch": secs / (steps // H) * 1e6,
                      "flop_per_launch": flops,
                      "algorithmic_bytes_per_launch": 32 * n},
     }
     extras = set() if args.no_extras else set(args.extras.split(",")) - {"none", ""}
-    sections = (("trapezoid", lambda: _trapezoid(sw, n, dt, rng_ic, args)),
+    sections = (("trapezoid", lambda: _trapezoid(sw, n, dt, u0, args)),
                 ("equations", lambda: _equations(sw, n)),
                 ("sizes", lambda: _sizes(sw, args.seed)),
                 ("batched", lambda: _batched(sw)),
@@ -180,8 +218,10 @@ def main():
             out[name] = run()
             _progress(name)
     if not args.no_cpu:
-        out["cpu_baseline"] = _cpu_baseline(n, dt, rng_ic, args.cpu_seconds)
+        out["cpu_baseline"] = _cpu_baseline(n, dt, u0, args.cpu_seconds)
         _progress("cpu_baseline")
+        out["error_vs_fp64_oracle"] = _oracle_check(sw, n, dt, u0, W, out["cpu_baseline"]["check_steps"])
+        _progress("oracle check")
     print(json.dumps(out))
 
 
@@ -241,7 +281,7 @@ def _sizes(sw, seed, steps=20000):
     res = {}
     for lg in (14, 16, 18, 20):
         n = 1 << lg
-        u0 = _ks_ic(n, seed)
+        u0 = sw.ks_noisy_ic(n, seed)
         row = {"swept": {}}
         for alg, name, widths in ((sw.CLASSIC, "classic", (256,)), (sw.ALT, "register", (128,)),
                                   (sw.SWEPT, "swept", (64, 128, 256, 512, 1024))):
@@ -289,7 +329,7 @@ def _sizes(sw, seed, steps=20000):
     res["euler_f64"] = eu
     # FP32 at the largest size: classic and the swept node widths that can win there
     n = 1 << 20
-    u0 = _ks_ic(n, seed).astype(np.float32)
+    u0 = sw.ks_noisy_ic(n, seed, precision=sw.F32)
     row = {}
     for alg, W, key in ((sw.CLASSIC, 256, "classic"), (sw.SWEPT, 512, "swept_512"), (sw.SWEPT, 1024, "swept_1024")):
         p = sw.Plan(sw.KS, n, 0.005, precision=sw.F32, algorithm=alg, tpb=W, node_width=W if alg == sw.SWEPT else 0)
@@ -304,15 +344,18 @@ def _sizes(sw, seed, steps=20000):
 
 
 def _trapezoid(sw, n, dt, u0, args):
-    """Same workload on the overlapped trapezoid engine (algorithm 3)."""
+    """Same workload on the overlapped trapezoid engine (algorithm 3), timed
+    over the same requested steps rounded up to whole tile launches."""
     res = {}
     for W, depth in ((512, 2), (1024, 2)):
         p = sw.Plan(sw.KS, n, dt, algorithm=sw.TRAPEZOID, node_width=W, tile_depth=depth)
+        H = p.info()["tile_steps"]
+        steps = _whole_launches(H, args.steps, MIN_TIMED_LAUNCHES)
         p.set_state(u0)
-        p.advance(args.warmup)
-        secs = _timed(p, args.steps)
+        _warm(p, args.warmup, MIN_WARM_LAUNCHES)
+        secs = _timed(p, steps)
         p.close()
-        res["W%d_depth%d" % (W, depth)] = n * args.steps / secs
+        res["W%d_depth%d" % (W, depth)] = n * steps / secs
     best = max(res, key=res.get)
     return {"value": res[best], "config": best, "all": res, "unit": "point-updates/s"}
 
@@ -366,14 +409,6 @@ def _batched(sw, count=4096, steps=5000, loop_sample=512):
     return res
 
 
-def _ks_ic(n, seed):
-    # same formula as the oracle's synthetic IC; bench needs no oracle for it
-    x = np.arange(n) * 0.5
-    base = 2.0 * np.cos(19.0 * x * np.pi / 128.0)
-    rng = np.random.default_rng(seed)
-    return base + 1e-3 * rng.standard_normal(n)
-
-
 def _surrogate():
     """Learned KS surrogate (nX=1024, K=20): trained on the GPU from 512 solver
     trajectories of 10000 steps, then a judged rollout (every 10th network
@@ -402,7 +437,9 @@ def _cpu_baseline(n, dt, u0, budget_s, repeats=5):
     host threads this job may use (at most 16), and one thread.  For each,
     the step count is set so that one run takes about budget_s / (2 repeats)
     seconds (at least 1 s), and the median rate of `repeats` runs is kept;
-    only the oracle call is timed (the state copy into it is ~1 ms)."""
+    only the oracle call is timed (the state copy into it is ~1 ms).
+    check_steps: the OpenMP runs' step count, which _oracle_check replays on
+    the GPU."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
 
@@ -434,7 +471,32 @@ def _cpu_baseline(n, dt, u0, budget_s, repeats=5):
                       f"C oracle (oracle/swept1d_oracle.c) with OpenMP over {threads} threads",
             "serial": {"value": one, "cores": 1,
                        "sample": f"same, {one_steps} steps per run, median of {repeats}, one thread"},
-            "host": {"cpu": _cpu_model(), "cpus_visible": os.cpu_count(), "cpus_allowed": avail}}
+            "host": {"cpu": _cpu_model(), "cpus_visible": os.cpu_count(), "cpus_allowed": avail},
+            "check_steps": omp_steps}
+
+
+def _oracle_check(sw, n, dt, u0, W, steps):
+    """The stated error of the headline (part of the CPU-baseline leg): the
+    swept plan of the timed configuration advances the same initial condition
+    by `steps` steps (the CPU baseline's run length, rounded up to whole tile
+    launches), and the OpenMP oracle's state after the same steps is the
+    reference it is compared with."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    with sw.Plan(sw.KS, n, dt, algorithm=sw.SWEPT, tpb=W, node_width=W) as p:
+        H = p.info()["tile_steps"]
+        steps = _whole_launches(H, steps, 2)
+        p.set_state(u0)
+        p.advance(steps)
+        got = p.get_state()
+    want = pyoracle.ks_run(u0, steps, pyoracle.ks_consts(dt), max(1, min(16, avail)))
+    diff = np.abs(got - want)
+    scale = float(np.max(np.abs(want)))
+    return {"steps": steps, "diamond_launches": max(steps // H - 1, 0), "bitwise_equal": bool(np.array_equal(got, want)),
+            "max_abs_diff": float(diff.max()), "max_rel_diff": float(diff.max() / scale) if scale else 0.0,
+            "tolerance_rel": 1e-12}
 
 
 def _cpu_model():

@@ -138,6 +138,20 @@ int sw1d_schedule(int32_t kind, double dt, double tf_arg, double freq, int32_t t
                   double dt_sched, int64_t* out_steps, double* out_times,
                   int64_t capacity, int64_t* count);
 
+/* Seeded initial conditions -------------------------------------------------
+ * The reference's initFun (KS1D_SweptShared.cu:68-72, 945; the heat IC of
+ * Heat1D_SweptShared.cu:60-66 is replaced by a periodic one) extended with the
+ * synthetic workloads' seeded terms, drawn on the host from splitmix64 so that
+ * every host gives the same values.  Computed in double, then rounded to REAL.
+ *   SW1D_IC_KS_NOISY      u_k = 2cos(19 pi x/128) + param * N(0,1), x = k * length
+ *                         (length = dx; Box-Muller, cosine branch)
+ *   SW1D_IC_HEAT_FOURIER  T_j = 1 + sum_{k=1..4} a_k sin(2 pi k x/L), a_k ~ U[0,1),
+ *                         x = j L / n (length = L; param unused)
+ * Host only: needs no GPU. */
+enum { SW1D_IC_KS_NOISY = 0, SW1D_IC_HEAT_FOURIER = 1 };
+int sw1d_initial_condition(int32_t kind, int32_t precision, int64_t n, double length, double param,
+                           uint64_t seed, void* host);
+
 /* Batched solves -------------------------------------------------------------
  * No reference counterpart (the reference solves one problem per process):
  * many independent periodic heat / KS problems of any sizes, advanced together

@@ -60,7 +60,6 @@ def _declare(L):
         getattr(L, "orc_euler_ic_ref" + s).argtypes = [_P, _I64, _P]
     L.orc_ic_heat_fourier.argtypes = [_P, _I64, _D, _U64, _P]
     L.orc_ic_ks_noisy.argtypes = [_P, _I64, _D, _D, _U64]
-    L.orc_ic_fourier_modes.argtypes = [_P, _I64, _D, _INT, _U64]
     L.orc_splitmix64_next.argtypes = [_P]
     L.orc_splitmix64_next.restype = _U64
 
@@ -126,12 +125,6 @@ def ic_heat_fourier(n, seed, L=1.0):
 def ic_ks_noisy(n, seed, dx=0.5, sigma=1e-3):
     u = np.zeros(n, np.float64)
     lib().orc_ic_ks_noisy(_ptr(u), n, dx, sigma, seed)
-    return u
-
-
-def ic_fourier_modes(n, seed, L, K=16):
-    u = np.zeros(n, np.float64)
-    lib().orc_ic_fourier_modes(_ptr(u), n, L, K, seed)
     return u
 
 

@@ -57,3 +57,27 @@ def test_null_arguments(sw):
     assert L.sw1d_advance(None, 1) == -1
     assert L.sw1d_get_state(None, None) == -1
     assert L.sw1d_plan_destroy(None) == 0
+
+
+@pytest.mark.parametrize("n,seed", [(1000, 1), (1 << 16, 7)])
+def test_initial_condition_equals_oracle(sw, orc, n, seed):
+    """sw1d_initial_condition is host-only code (no GPU): its seeded fields are
+    the oracle's bit for bit, rounded once to float in single precision."""
+    import numpy as np
+
+    want = orc.ic_ks_noisy(n, seed)
+    assert np.array_equal(sw.ks_noisy_ic(n, seed), want)
+    assert np.array_equal(sw.ks_noisy_ic(n, seed, precision=sw.F32), want.astype(np.float32))
+    for L in (1.0, n * 0.001):
+        want, _ = orc.ic_heat_fourier(n, seed, L=L)
+        assert np.array_equal(sw.heat_fourier_ic(n, seed, L=L), want)
+
+
+def test_initial_condition_validation(sw):
+    import numpy as np
+
+    out = np.empty(4)
+    L = sw.lib()
+    assert L.sw1d_initial_condition(9, 0, 4, 0.5, 0.0, 1, out.ctypes.data_as(C.c_void_p)) == -1
+    assert L.sw1d_initial_condition(0, 0, 0, 0.5, 0.0, 1, out.ctypes.data_as(C.c_void_p)) == -1
+    assert L.sw1d_initial_condition(0, 0, 4, 0.5, 0.0, 1, None) == -1

@@ -20,8 +20,24 @@ def test_cpu_baseline_fields():
     sys.path.insert(0, ROOT)
     import bench
 
-    r = bench._cpu_baseline(4096, 0.005, bench._ks_ic(4096, 1), 0.5, repeats=3)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+
+    r = bench._cpu_baseline(4096, 0.005, pyoracle.ic_ks_noisy(4096, 1), 0.5, repeats=3)
     assert r["kind"] == "port" and r["unit"] == "point-updates/s"
     assert r["value"] > 0 and r["serial"]["value"] > 0 and r["serial"]["cores"] == 1
     assert 1 <= r["cores"] <= 16 and r["cores"] <= r["host"]["cpus_allowed"]
     assert "median of 3" in r["sample"] and r["host"]["cpu"]
+    assert r["check_steps"] >= 2
+
+
+def test_whole_launch_rounding():
+    """Timed and warm-up steps are whole tile launches, at least the minimum:
+    the driver's --steps 20 gives 160 launches of the 128-step diamond."""
+    sys.path.insert(0, ROOT)
+    import bench
+
+    assert bench._whole_launches(128, 20, 160) == 160 * 128
+    assert bench._whole_launches(128, 20001, 2) == 157 * 128
+    assert bench._whole_launches(128, 128 * 200, 160) == 200 * 128
+    assert bench._whole_launches(1, 20, 40) == 40

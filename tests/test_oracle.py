@@ -59,13 +59,57 @@ def test_euler_reference_ic_and_constants(orc):
     assert (q[: n // 2] == bd[:3]).all() and (q[n // 2:] == bd[3:]).all()
 
 
-def test_heat_fourier_ic(orc):
+def _uniforms(orc, seed, count):
+    """The generator's uniform draws: the top 53 bits of each splitmix64 output."""
+    return np.array([(v >> 11) * 2.0 ** -53 for v in orc.splitmix64_stream(seed, count)])
+
+
+@pytest.mark.parametrize("seed,L", [(9, 1.0), (5, 1.024), (123456789, 3.5)])
+def test_heat_fourier_ic(orc, seed, L):
+    """The amplitudes are the first four uniform draws of the pinned generator,
+    and a DFT of the field recovers exactly them as sine modes 1..4 (any wrong
+    mode number, phase, wavelength or offset leaves other coefficients)."""
     n = 1024
-    u, amp = orc.ic_heat_fourier(n, 9)
-    x = np.arange(n) / n
-    want = 1 + sum(amp[k] * np.sin(2 * np.pi * (k + 1) * x) for k in range(4))
-    assert np.allclose(u, want, atol=1e-14)
-    assert ((amp >= 0) & (amp < 1)).all()
+    u, amp = orc.ic_heat_fourier(n, seed, L=L)
+    assert np.array_equal(amp, _uniforms(orc, seed, 4))
+    c = np.fft.rfft(u) / n
+    assert abs(c[0] - 1.0) < 1e-14
+    for k in range(1, 5):  # a sin(2 pi k j / n) has DFT coefficient -i a / 2
+        assert abs(c[k].real) < 1e-14 and abs(-2 * c[k].imag - amp[k - 1]) < 1e-14
+    assert np.max(np.abs(c[5:])) < 1e-14
+
+
+def test_heat_fourier_amplitudes_are_uniform(orc):
+    """Over many seeds the amplitudes fill [0, 1) uniformly (Kolmogorov-Smirnov)."""
+    from scipy import stats
+
+    amps = np.concatenate([orc.ic_heat_fourier(8, s)[1] for s in range(2000)])
+    assert ((amps >= 0) & (amps < 1)).all()
+    assert stats.kstest(amps, "uniform").pvalue > 1e-3
+
+
+def test_ks_noise_is_gaussian_with_stated_sigma(orc):
+    """u - 2cos(19 pi x/128) at 2^20 points is N(0, 1e-3) noise: sample mean
+    within 5 standard errors of 0, standard deviation within 0.5% of 1e-3
+    (7 standard errors), a Kolmogorov-Smirnov test against N(0, 1e-3) and no
+    lag-1 correlation.  A wrong scale, a missing sqrt(-2 ln u1), a biased or
+    reused uniform, or the sine in place of the cosine of the base field
+    fails one of them."""
+    from scipy import stats
+
+    n, sigma = 1 << 20, 1e-3
+    u = orc.ic_ks_noisy(n, 1)
+    x = np.arange(n) * 0.5
+    r = u - 2.0 * np.cos(19.0 * x * np.pi / 128.0)
+    assert abs(r.mean()) < 5e-6
+    assert abs(r.std() / sigma - 1.0) < 5e-3
+    assert stats.kstest(r / sigma, "norm").pvalue > 1e-3
+    assert abs(np.corrcoef(r[:-1], r[1:])[0, 1]) < 5.0 / np.sqrt(n)
+    # two uniform draws per point, in the Box-Muller cosine form
+    u1, u2 = _uniforms(orc, 1, 2)
+    assert abs(r[0] - sigma * np.sqrt(-2 * np.log(1 - u1)) * np.cos(2 * np.pi * u2)) < 1e-15
+    base = 2.0 * np.cos(19.0 * np.arange(16) * 0.5 * np.pi / 128.0)
+    assert np.max(np.abs(orc.ic_ks_noisy(16, 1, sigma=0.0) - base)) < 1e-15
 
 
 # ------------------------------------------------------------------- heat ---
