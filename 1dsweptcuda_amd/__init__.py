@@ -367,6 +367,7 @@ class Surrogate:
         L.sw1d_nn_fit.argtypes = [P, C.c_int64, C.c_int32, C.c_int64, C.c_float, C.c_uint64,
                                   C.POINTER(C.c_double)]
         L.sw1d_nn_eval.argtypes = [P, C.c_int64, C.c_int64, C.POINTER(C.c_double)]
+        L.sw1d_nn_dataset_samples.argtypes = [P, C.c_int64, C.c_int64, P, P]
         self.nx, self.max_batch = nx, max_batch
         self._h = C.c_void_p()
         self._chk(L.sw1d_nn_create(C.byref(self._h), nx, max_batch, seed), "sw1d_nn_create")
@@ -450,6 +451,15 @@ class Surrogate:
         sc = (C.c_double * 2)()
         self._chk(lib().sw1d_nn_dataset(self._h, batch._h, nsnap, k_steps, C.byref(n), sc), "sw1d_nn_dataset")
         return n.value, (sc[0], sc[1])
+
+    def dataset_samples(self, first: int, count: int):
+        """Stored training pairs [first, first+count): (x, t), each [count, nx]
+        float32; sample s*B + b is snapshot s of problem b."""
+        x = np.empty((count, self.nx), np.float32)
+        t = np.empty((count, self.nx), np.float32)
+        self._chk(lib().sw1d_nn_dataset_samples(self._h, first, count, x.ctypes.data_as(C.c_void_p),
+                                                t.ctypes.data_as(C.c_void_p)), "sw1d_nn_dataset_samples")
+        return x, t
 
     def fit(self, iters: int, batch: int = 32, lr: float = 1e-3, nfit: int = 0, seed: int = 0) -> float:
         loss = C.c_double(0)

@@ -27,6 +27,7 @@
 // (0 double, 1 float), int64 nX, int64 values per point (1, or 3 for Euler),
 // double time, double dt, then nX * values REALs in the reference's layout.
 
+#include <algorithm>
 #include <charconv>
 #include <chrono>
 #include <cmath>
@@ -363,28 +364,57 @@ int main(int argc, char* argv[])
 
 #if SW1D_CLI_EQ == 1
     if (scheme > 2) {
-        // Register-vs-shared self test (KS:682-750): both swept engines from the
-        // same initial condition over one register cycle, compared with tol dt^3.
+        // Register-vs-shared self test (KS:641-656, 682-750).  The shared
+        // engine (alg 1: node width tpb, the node image in LDS from 256-point
+        // nodes up) and the register engine (alg 2: 128-point nodes held in
+        // VGPRs) advance the same initial condition, and each phase's state is
+        // compared with the reference's tolerance dt^3.  Every phase is a
+        // whole number of tiles of both engines, so both run swept tiles only:
+        // "UP/DOWN" is one tile of the engine with the taller tile (its
+        // up-triangle, then its down-triangle), "WHOLE" three of them, which
+        // puts whole and boundary-straddling diamonds in both.  The launches
+        // of each kind are printed per engine.
         cout << endl << "--- TEST IT ----" << endl << endl << endl;
         sw1d_problem a = prob, b = prob;
         a.algorithm = SW1D_SWEPT;
         b.algorithm = SW1D_ALT;
         sw1d_plan *pa = nullptr, *pb = nullptr;
         if (sw1d_plan_create(&pa, &a) || sw1d_plan_create(&pb, &b)) die_lib("plan");
-        vector<REAL> ra(IC), rb(IC);
-        if (sw1d_set_state(pa, IC.data()) || sw1d_set_state(pb, IC.data()) || sw1d_advance(pa, 8) ||
-            sw1d_advance(pb, 8) || sw1d_get_state(pa, ra.data()) || sw1d_get_state(pb, rb.data()))
-            die_lib("self test");
+        int64_t ia[5], ib[5];
+        if (sw1d_plan_info(pa, ia) || sw1d_plan_info(pb, ib)) die_lib("plan");
+        const int64_t hmax = std::max(ia[1], ib[1]);
         const REAL tol = (REAL)(dt * dt * dt);
         cout << "Tolerance: " << tol << endl;
-        cout << "---- TEST DOWN --------" << endl;
-        int t = 0;
-        for (int k = 0; k < dv; k++) t += (fabs(ra[k] - rb[k]) > tol);
-        if (!t) {
-            cout << "EQUAL!" << endl;
-        } else {
-            cout << "NOT EQUAL!" << endl;
-            cout << t << " out of " << dv << endl;
+        vector<REAL> ra(IC), rb(IC);
+        const char* kind_names[5] = {"classic", "up", "down", "diamond", "split"};
+        const struct { const char* name; int64_t steps; } phases[2] = {{"UP/DOWN", hmax}, {"WHOLE", 3 * hmax}};
+        for (const auto& ph : phases) {
+            sw1d_plan* pl[2] = {pa, pb};
+            vector<REAL>* res[2] = {&ra, &rb};
+            int64_t counts[2][5] = {};
+            for (int e = 0; e < 2; ++e) {
+                double ms = 0;
+                if (sw1d_set_state(pl[e], IC.data()) || sw1d_set_profiling(pl[e], 1) ||
+                    sw1d_advance(pl[e], ph.steps) || sw1d_get_state(pl[e], res[e]->data()))
+                    die_lib("self test");
+                for (int k = 0; k < 5; ++k)
+                    if (sw1d_kernel_stats(pl[e], k, &counts[e][k], &ms)) die_lib("self test");
+                if (sw1d_set_profiling(pl[e], 0)) die_lib("self test");
+            }
+            cout << "---- TEST " << ph.name << " (" << ph.steps << " steps) --------" << endl;
+            for (int e = 0; e < 2; ++e) {
+                cout << (e ? "register" : "shared") << " node " << (e ? ib[0] : ia[0]) << ":";
+                for (int k = 0; k < 5; ++k) cout << " " << kind_names[k] << " " << counts[e][k];
+                cout << endl;
+            }
+            int t = 0;
+            for (int k = 0; k < dv; k++) t += (fabs(ra[k] - rb[k]) > tol);
+            if (!t) {
+                cout << "EQUAL!" << endl;
+            } else {
+                cout << "NOT EQUAL!" << endl;
+                cout << t << " out of " << dv << endl;
+            }
         }
         sw1d_plan_destroy(pa);
         sw1d_plan_destroy(pb);

@@ -1628,6 +1628,22 @@ int sw1d_nn_eval(sw1d_nn* N, int64_t first, int64_t count, double* rel_err)
     return SW1D_OK;
 }
 
+// Copy stored pairs [first, first+count) to the host (sample s*B + b is
+// snapshot s of problem b).  x_host / t_host may be NULL.
+int sw1d_nn_dataset_samples(sw1d_nn* N, int64_t first, int64_t count, float* x_host, float* t_host)
+{
+    NnData* D = N ? data_of(N) : nullptr;
+    if (!D || !D->x) return nn_fail(SW1D_EINVAL, "no dataset (call sw1d_nn_dataset first)");
+    if (first < 0 || count < 1 || first + count > D->nsamples) return nn_fail(SW1D_EINVAL, "range");
+    const size_t row = (size_t)N->X, bytes = (size_t)count * row * sizeof(float);
+    if (x_host)
+        NN_HIP(hipMemcpyAsync(x_host, D->x + first * row, bytes, hipMemcpyDeviceToHost, N->stream));
+    if (t_host)
+        NN_HIP(hipMemcpyAsync(t_host, D->t + first * row, bytes, hipMemcpyDeviceToHost, N->stream));
+    NN_HIP(hipStreamSynchronize(N->stream));
+    return SW1D_OK;
+}
+
 void sw1d_nn_release_dataset(sw1d_nn* N) { free_data(N); }
 
 }  // extern "C"

@@ -90,17 +90,30 @@ def evaluate(nn, ntraj=256, warm=2000, network_steps=200, judge_every=10, tol=1e
         pu = float(ntraj) * NX * K * network_steps
         rep["judged_rollout"] = st
         rep["surrogate_pu_per_s"] = pu / secs
+        judged = b.get_state().reshape(ntraj, NX)
         b.set_state(u0)
         t0 = time.perf_counter()
         nn.advance(b, network_steps, K, judge_every=0)
         secs_nj = time.perf_counter() - t0
         rep["surrogate_unjudged_pu_per_s"] = pu / secs_nj
+        unjudged = b.get_state().reshape(ntraj, NX)
         # plain batched solver, same problems and horizon (state stays in HBM)
         ref.advance(K * network_steps)
         ref.set_state(u0)
         t0 = time.perf_counter()
         ref.advance_timed(K * network_steps)
         rep["plain_batch_pu_per_s"] = pu / (time.perf_counter() - t0)
+        # accumulated error of the whole rollout against the plain solver's
+        # trajectory from the same states, per problem (KS is chaotic: the
+        # distance of two trajectories grows with the horizon whatever its start)
+        plain_end = ref.get_state().reshape(ntraj, NX)
+        norm = np.linalg.norm(plain_end, axis=1)
+        rep["rollout_rel_l2"] = {
+            "horizon_solver_steps": K * network_steps,
+            "judged": {"median": float(np.median(np.linalg.norm(judged - plain_end, axis=1) / norm)),
+                       "max": float(np.max(np.linalg.norm(judged - plain_end, axis=1) / norm))},
+            "unjudged": {"median": float(np.median(np.linalg.norm(unjudged - plain_end, axis=1) / norm)),
+                         "max": float(np.max(np.linalg.norm(unjudged - plain_end, axis=1) / norm))}}
         # error of the unjudged network after one network step, per trajectory
         ref.set_state(u0)
         ref.advance(K)

@@ -420,15 +420,18 @@ def _surrogate():
     nn, rep = sur.train(ntraj=512, steps=10000, iters=3000, log=lambda *_: None)
     ev = sur.evaluate(nn, ntraj=256)
     nn.close()
-    return {"held_out_rel_err_du": rep["held_out_rel_err_du"], "train_seconds": rep["train_seconds"],
-            "dataset_samples": rep["samples"], "judged_rollout": ev["judged_rollout"],
-            "one_step_rel_l2": ev["one_step_rel_l2"],
-            "value": ev["surrogate_pu_per_s"], "unit": "point-update equivalents/s",
+    return {"value": ev["surrogate_pu_per_s"], "unit": "point-update equivalents/s",
+            "speedup_vs_plain_batch": ev["speedup_vs_plain_batch"],
+            "plain_batch_same_problems": ev["plain_batch_pu_per_s"],
+            "note": "like for like: the plain batched solver on the same 256 problems and horizon; "
+                    "the network costs ~8.2k flop per point-update equivalent against KS's 40",
+            "judged_rollout": ev["judged_rollout"], "tolerance_per_judged_step": 1e-2,
+            "one_step_rel_l2": ev["one_step_rel_l2"], "rollout_rel_l2": ev["rollout_rel_l2"],
+            "held_out_rel_err_du": rep["held_out_rel_err_du"], "train_seconds": rep["train_seconds"],
+            "dataset_samples": rep["samples"],
             "unjudged_value": ev["surrogate_unjudged_pu_per_s"],
             "plain_single_swept_nx1024": ev["plain_single_swept_pu_per_s"],
-            "plain_batch_same_problems": ev["plain_batch_pu_per_s"],
             "speedup_vs_single_swept": ev["speedup_vs_single_swept"],
-            "speedup_vs_plain_batch": ev["speedup_vs_plain_batch"],
             "seconds": time.perf_counter() - t0}
 
 

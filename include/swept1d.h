@@ -108,7 +108,7 @@ int sw1d_advance_timed(sw1d_plan* plan, int64_t nsteps, double* elapsed_ms);
 
 /* Plan facts ----------------------------------------------------------------
  * info[0] node width (points per swept node), info[1] steps advanced by one
- * swept tile launch, info[2] values stored per point (1 heat, 2 KS, 6 Euler),
+ * swept tile launch, info[2] values stored per point (2 heat and KS, 6 Euler),
  * info[3] device bytes held by the plan, info[4] steps per reference cycle
  * (the t_fullstep of sweptWrapper / classicWrapper divided by dt). */
 int sw1d_plan_info(const sw1d_plan* plan, int64_t info[5]);
@@ -217,6 +217,11 @@ int sw1d_nn_train(sw1d_nn* nn, const float* d_inputs, const float* d_targets, in
  * the K-step change (written to scales[0..1] and set on the network). */
 int sw1d_nn_dataset(sw1d_nn* nn, sw1d_batch* batch, int32_t nsnap, int32_t k_steps,
                     int64_t* nsamples, double* scales);
+/* Copies stored pairs [first, first+count) to the host: sample s*B + b is
+ * snapshot s of problem b of the batch, nx floats each (x = u_s / in_scale,
+ * t = (u_{s+1} - u_s) / out_scale, each computed in double, rounded once).
+ * x_host or t_host may be NULL. */
+int sw1d_nn_dataset_samples(sw1d_nn* nn, int64_t first, int64_t count, float* x_host, float* t_host);
 /* Adam on the first nfit samples of the stored training set */
 int sw1d_nn_fit(sw1d_nn* nn, int64_t nfit, int32_t batch, int64_t iters, float lr, uint64_t seed,
                 double* last_loss);

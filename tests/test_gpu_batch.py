@@ -78,3 +78,71 @@ def test_batch_workgroup_setting(sw, orc, wl):
         else:
             want = orc.ks_run(ics[i], 144, orc.ks_consts(dt[i], dx=dx[i]))
         assert np.array_equal(got[i], want), i
+
+
+def _stratified(eq, nx, per_class=6):
+    """Indices covering every size class floor(log2 nX) and both equations,
+    `per_class` each (the largest of the class first), plus the largest and
+    smallest problem of the whole batch."""
+    cls = np.floor(np.log2(nx)).astype(int)
+    pick = {int(np.argmax(nx)), int(np.argmin(nx))}
+    for c in np.unique(cls):
+        for e in (0, 1):
+            idx = np.flatnonzero((cls == c) & (eq == e))
+            idx = idx[np.argsort(-nx[idx], kind="stable")]
+            pick.update(int(i) for i in idx[:per_class])
+    return sorted(pick)
+
+
+@pytest.fixture(scope="module")
+def c4_workload(wl):
+    eq, nx, dt, dx = wl.batch_problems(4096, seed=2024)
+    return eq, nx, dt, dx, wl.batch_ics(eq, nx, dx)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_batch_workload_full_shape(sw, orc, wl, c4_workload, prec):
+    """The batched workload at its own shape: 4096 heat/KS problems of
+    nX = floor(2^U[10,16]), 5000 steps, the tuned tile widths and waves per
+    workgroup, in one persistent launch.  The grid holds fewer workgroups than
+    problems, so workgroups pull further problems from the queue.  A sample
+    covering every size class and both equations (>= 64 problems, with the
+    largest) is bitwise the oracle's."""
+    import os
+    from concurrent.futures import ThreadPoolExecutor
+
+    import importlib
+
+    at = importlib.import_module("1dsweptcuda_amd.autotune")
+    eq, nx, dt, dx, ics64 = c4_workload
+    steps = 5000
+    dtype = np.float64 if prec == 0 else np.float32
+    table = at.load()
+    assert table, "the tuned tile table is missing"
+    tps = at.tile_points_for(nx, table)
+    waves = at.load_waves()
+    ics = [u.astype(dtype) for u in ics64]
+    print(f"batch C4 prec={prec} problems={nx.size} points={int(nx.sum())} steps={steps} waves={waves}",
+          flush=True)
+    with sw.Batch(eq, nx, dt, dx, precision=prec, tile_points=tps, waves=waves) as b:
+        info = b.info()
+        assert info["problems"] == 4096 and info["total_points"] == int(nx.sum())
+        assert info["workgroups"] < nx.size
+        b.set_state(np.concatenate(ics))
+        b.advance(steps)
+        got = b.split(b.get_state())
+    sample = _stratified(eq, nx)
+    assert len(sample) >= 64 and int(np.argmax(nx)) in sample
+    assert set(np.floor(np.log2(nx[sample])).astype(int)) == set(range(10, 16))
+
+    def oracle(i):
+        if eq[i] == 0:
+            fo, fo2 = orc.heat_consts(dt[i], dtype, ds=dx[i])
+            return orc.heat_run(ics[i], steps, fo, fo2, 1)
+        return orc.ks_run(ics[i], steps, orc.ks_consts(dt[i], dtype, dx=dx[i]))
+
+    workers = max(1, min(16, len(os.sched_getaffinity(0))))
+    with ThreadPoolExecutor(workers) as pool:
+        wants = list(pool.map(oracle, sample))
+    bad = [(i, int(eq[i]), int(nx[i])) for i, w in zip(sample, wants) if not np.array_equal(got[i], w)]
+    assert not bad, bad

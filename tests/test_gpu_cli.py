@@ -77,12 +77,15 @@ def _oracle_states(orc, eq, prec, n, dt, steps_list):
     return out
 
 
-CLI_CASES = [(eq, prec, alg) for eq in (0, 1, 2) for prec in (0, 1) for alg in (0, 1, 2)]
+CLI_CASES = [(eq, prec, alg, 128) for eq in (0, 1, 2) for prec in (0, 1) for alg in (0, 1, 2)]
+# tpb 64, the smallest the reference's timing sweep (runTest.sh) runs: 64-point
+# nodes, one point per lane, for the swept and hybrid/register algorithms
+CLI_CASES += [(eq, prec, alg, 64) for eq in (0, 1, 2) for prec in (0, 1) for alg in (1, 2)]
 
 
-@pytest.mark.parametrize("eq,prec,alg", CLI_CASES)
-def test_program_rows_equal_oracle(sw, orc, tmp_path, eq, prec, alg):
-    n, tpb = 1024, 128
+@pytest.mark.parametrize("eq,prec,alg,tpb", CLI_CASES)
+def test_program_rows_equal_oracle(sw, orc, tmp_path, eq, prec, alg, tpb):
+    n = 1024
     if eq == 0:
         dt, tf, freq = 0.001, 1.5, 0.5
     elif eq == 1:
@@ -147,9 +150,30 @@ def test_error_exits(sw, tmp_path):
     assert r.returncode == 255 and "too high" in r.stdout
 
 
-def test_ks_register_self_test(sw, tmp_path):
-    """alg 3: register vs shared swept self test prints EQUAL! and exits -1
-    (KS:682-750)."""
-    r = _run(sw, 1, 0, [1024, 256, 0.001, 1, 1, 3, "x.dat"], tmp_path)
-    assert r.returncode == 255
-    assert "EQUAL!" in r.stdout and "NOT EQUAL" not in r.stdout
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("tpb", [64, 256, 1024])
+def test_ks_register_self_test(sw, tmp_path, prec, tpb):
+    """alg 3: the register engine (128-point nodes in VGPRs) against the
+    shared engine (node width tpb) phase by phase, as KS:682-750 compares the
+    register kernels with the shared ones launch by launch: both engines run
+    swept tiles only (no classic step), the "WHOLE" phase puts diamonds in
+    both, every phase prints EQUAL!, and the program exits -1."""
+    r = _run(sw, 1, prec, [4096, tpb, 0.001, 1, 1, 3, "x.dat"], tmp_path)
+    assert r.returncode == 255, r.stdout
+    out = r.stdout
+    assert out.count("EQUAL!") == 2 and "NOT EQUAL" not in out
+    assert "---- TEST UP/DOWN" in out and "---- TEST WHOLE" in out
+    counts = {}
+    for line in out.splitlines():
+        if line.startswith(("shared node", "register node")):
+            head, rest = line.split(":")
+            v = rest.split()
+            counts.setdefault(head.split()[0], []).append(dict(zip(v[::2], map(int, v[1::2]))))
+    assert set(counts) == {"shared", "register"}
+    for engine, phases in counts.items():
+        updown, whole = phases
+        for ph in phases:
+            assert ph["classic"] == 0 and ph["up"] == 1 and ph["down"] == 1, (engine, ph)
+        assert whole["diamond"] + whole["split"] >= 2, (engine, whole)
+    # the engine with the taller tile runs no diamond in UP/DOWN, only in WHOLE
+    assert min(p[0]["diamond"] + p[0]["split"] for p in counts.values()) == 0

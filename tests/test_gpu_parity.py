@@ -96,21 +96,25 @@ def test_ks_phased_diamond_single_precision(sw, orc, W):
     assert np.array_equal(got, _oracle(orc, 1, u0, nsteps, dt, np.float32, 0, n))
 
 
-@pytest.mark.parametrize("eq", [0, 1, 2])
-@pytest.mark.parametrize("W", [64, 128, 256, 512, 1024])
-def test_node_widths(sw, orc, eq, W):
-    """Every node width the equation allows, 64 included: one point per lane,
-    where a radius-2 stencil's outer neighbour comes from two lanes away."""
-    if eq == 2 and W > 512:
-        pytest.skip("Euler nodes stop at 512 points")
+NODE_CASES = [(eq, W, prec, alg) for eq in (0, 1, 2) for W in (64, 128, 256, 512, 1024)
+              for prec in (0, 1) for alg in (1, 2) if not (eq == 2 and W > 512)]  # Euler nodes stop at 512
+
+
+@pytest.mark.parametrize("eq,W,prec,alg", NODE_CASES)
+def test_node_widths(sw, orc, eq, W, prec, alg):
+    """Every node width the equation allows, 64 included (one point per lane,
+    where a radius-2 stencil's outer neighbour comes from two lanes away), in
+    both precisions, with the shared swept algorithm and with algorithm 2 at
+    the same width (heat and Euler: the hybrid, whose boundary node the host
+    computes; KS: the register engine)."""
     n = 8 * W
-    dtype = np.float64
+    dtype = np.float64 if prec == 0 else np.float32
     dt = _dt(eq, n, 0)
     H = W // 2 if eq == 0 else W // 8
     nsteps = 5 * H + 3
-    _say(f"node width eq={eq} W={W} n={n} steps={nsteps}")
+    _say(f"node width eq={eq} W={W} prec={prec} alg={alg} n={n} steps={nsteps}")
     u0 = _ic(orc, eq, n, dtype, 0)
-    with sw.Plan(eq, n, dt, algorithm=1, node_width=W) as p:
+    with sw.Plan(eq, n, dt, precision=prec, algorithm=alg, tpb=W, node_width=W) as p:
         assert p.info()["node_width"] == W
         p.set_state(u0)
         p.advance(nsteps)
@@ -119,24 +123,24 @@ def test_node_widths(sw, orc, eq, W):
     assert np.array_equal(got, want)
 
 
-@pytest.mark.parametrize("eq,lg", [(0, 14), (1, 14), (1, 18), (2, 16)])
-def test_auto_node_width(sw, orc, eq, lg):
+@pytest.mark.parametrize("eq,lg,prec", [(0, 14, 0), (1, 14, 0), (1, 18, 0), (2, 16, 0),
+                                         (1, 14, 1), (1, 18, 1), (2, 14, 1), (0, 16, 1)])
+def test_auto_node_width(sw, orc, eq, lg, prec):
     """node_width="auto" takes the autotuner's width for the equation, precision
     and size class (tuned/node_widths.json) and gives the oracle's bits, as
     every width does."""
     import importlib
     at = importlib.import_module("1dsweptcuda_amd.autotune")
     n = 1 << lg
-    W = at.node_width_for(eq, n, 0)
-    if W == 0:
-        pytest.skip("no tuned node width for this equation")
-    dtype = np.float64
+    W = at.node_width_for(eq, n, prec)
+    assert W > 0, "no tuned node width for this equation and precision"
+    dtype = np.float64 if prec == 0 else np.float32
     dt = _dt(eq, n, 0)
     H = W // 2 if eq == 0 else W // 8
     nsteps = 2 * H + 5
-    _say(f"auto node width eq={eq} n={n} W={W} steps={nsteps}")
+    _say(f"auto node width eq={eq} n={n} prec={prec} W={W} steps={nsteps}")
     u0 = _ic(orc, eq, n, dtype, 0)
-    with sw.Plan(eq, n, dt, algorithm=1, node_width="auto") as p:
+    with sw.Plan(eq, n, dt, precision=prec, algorithm=1, node_width="auto") as p:
         assert p.info()["node_width"] == W
         p.set_state(u0)
         p.advance(nsteps)

@@ -175,3 +175,75 @@ def test_training_reduces_error_and_judge(sw, wl):
             assert np.array_equal(b1.get_state(), b2.get_state())
             st = nn.advance(b1, 5, K, judge_every=10, tol=1e-2)
             assert st["judged"] == 0 and st["network_steps"] == 5
+
+
+def test_dataset_pairs_and_scales_match_oracle(sw, orc, wl):
+    """sw1d_nn_dataset: snapshots every K steps of an FP64 KS batch; in_scale
+    and out_scale are the RMS of u and of the K-step change over the
+    snapshots, and each stored pair is (u_s / in_scale, (u_{s+1} - u_s) /
+    out_scale) rounded once to float.  Snapshots from the oracle's KS run."""
+    B, nx, K, nsnap, lead = 4, 1024, 20, 5, 100
+    ics = [wl.fourier_ic(nx, nx * 0.5, 300 + i) for i in range(B)]
+    c = orc.ks_consts(0.005)
+    snaps = []  # [nsnap + 1][B][nx]
+    u = [orc.ks_run(u0, lead, c) for u0 in ics]
+    snaps.append(np.array(u))
+    for _ in range(nsnap):
+        u = [orc.ks_run(v, K, c) for v in u]
+        snaps.append(np.array(u))
+    snaps = np.array(snaps)
+    d = np.diff(snaps, axis=0)
+    in_scale = np.sqrt(np.mean(snaps[:-1] ** 2))
+    out_scale = np.sqrt(np.mean(d ** 2))
+    print("surrogate dataset B=4 nsnap=5", flush=True)
+    with sw.Surrogate(nx, 8, seed=0) as nn, sw.Batch(np.ones(B, np.int32), np.full(B, nx, np.int64),
+                                                      np.full(B, 0.005), np.full(B, 0.5)) as b:
+        b.set_state(np.concatenate(ics))
+        b.advance(lead)
+        n, scales = nn.dataset(b, nsnap, K)
+        assert n == nsnap * B
+        x, t = nn.dataset_samples(0, n)
+    assert abs(scales[0] / in_scale - 1) < 1e-12 and abs(scales[1] / out_scale - 1) < 1e-12
+    for s in range(nsnap):
+        for p in range(B):
+            k = s * B + p
+            assert np.array_equal(x[k], (snaps[s, p] * (1.0 / scales[0])).astype(np.float32)), (s, p)
+            assert np.array_equal(t[k], ((snaps[s + 1, p] - snaps[s, p]) * (1.0 / scales[1])).astype(np.float32))
+
+
+def test_judge_keeps_some_and_replaces_others(sw, wl):
+    """A tolerance between the smallest and largest per-problem error of one
+    network step: each problem then holds either the network's state (error
+    within tolerance) or the plain solver's (fallback), bit for bit, and the
+    fallback count and largest judged error agree with the distances."""
+    B, nx, K = 8, 1024, 20
+    rng = np.random.default_rng(21)
+    p = (rng.standard_normal(sw.Surrogate.param_count()) * 0.05).astype(np.float32)
+    # uneven amplitudes give uneven relative errors
+    u0 = np.concatenate([(0.3 + 0.4 * i) * wl.fourier_ic(nx, nx * 0.5, 500 + i) for i in range(B)])
+    args = (np.ones(B, np.int32), np.full(B, nx, np.int64), np.full(B, 0.005), np.full(B, 0.5))
+    print("surrogate mixed judge B=8", flush=True)
+    with sw.Surrogate(nx, B, seed=0) as nn, sw.Batch(*args) as b, sw.Batch(*args) as ref:
+        nn.set_params(p)
+        nn.set_scales(1.0, 0.02)
+        b.set_state(u0)
+        nn.advance(b, 1, K, judge_every=0)
+        net = b.get_state().reshape(B, nx)
+        ref.set_state(u0)
+        ref.advance(K)
+        plain = ref.get_state().reshape(B, nx)
+        e = np.linalg.norm(net - plain, axis=1) / np.linalg.norm(plain, axis=1)
+        order = np.sort(e)
+        assert order[0] < order[-1]
+        gaps = np.diff(order)
+        j = int(np.argmax(gaps[1:-1])) + 1 if B > 3 else int(np.argmax(gaps))
+        tol = 0.5 * (order[j] + order[j + 1])  # far from every distance
+        b.set_state(u0)
+        st = nn.advance(b, 1, K, judge_every=1, tol=tol)
+        got = b.get_state().reshape(B, nx)
+    kept = e <= tol
+    assert 0 < kept.sum() < B
+    assert st["judged"] == 1 and st["fallbacks"] == int((~kept).sum())
+    assert abs(st["max_judged_error"] / e.max() - 1) < 1e-9
+    for i in range(B):
+        assert np.array_equal(got[i], net[i] if kept[i] else plain[i]), (i, e[i], tol)
