@@ -104,6 +104,8 @@ hipError_t launch_classic(int eq, int bc, bool f64, int stage, int64_t n, const 
 // Checks the lane-shift primitive the swept kernels rely on; returns 1 if
 // every lane received its neighbour's value, 0 otherwise, <0 on HIP error.
 int check_lane_shifts();
+// The same check on caller-provided device memory (one int; no allocation).
+int check_lane_shifts_at(int* d);
 
 // Points per lane of a swept node (node width = 64 x that).  One is enough
 // for every stencil (a radius-2 neighbour two lanes away arrives by a second

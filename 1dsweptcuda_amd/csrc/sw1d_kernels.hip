@@ -1008,10 +1008,8 @@ hipError_t launch_classic(int eq, int bc, bool f64, int stage, int64_t n, const 
                : classic_prec<float>(eq, bc, stage, n, in, out, c, s);
 }
 
-int check_lane_shifts()
+int check_lane_shifts_at(int* d)
 {
-    int* d = nullptr;
-    if (hipMalloc(&d, sizeof(int)) != hipSuccess) return -1;
     const int one = 1;
     int ok = 0;
     hipError_t e = hipMemcpy(d, &one, sizeof(int), hipMemcpyHostToDevice);
@@ -1020,8 +1018,16 @@ int check_lane_shifts()
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpy(&ok, d, sizeof(int), hipMemcpyDeviceToHost);
-    (void)hipFree(d);
     return e == hipSuccess ? ok : -1;
+}
+
+int check_lane_shifts()
+{
+    int* d = nullptr;
+    if (hipMalloc(&d, sizeof(int)) != hipSuccess) return -1;
+    const int r = check_lane_shifts_at(d);
+    (void)hipFree(d);
+    return r;
 }
 
 }  // namespace sw1d

@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -622,6 +623,128 @@ hipError_t euler_layout(sw1d_plan* p, const void* in, void* out, bool split)
     return hipGetLastError();
 }
 
+// Validation, constants and tile geometry of a problem: everything a plan
+// holds except its stream and memory.  Allocates nothing and makes no device
+// call.
+int plan_setup(sw1d_plan* p, const sw1d_problem* q)
+{
+    if (q->equation < SW1D_HEAT || q->equation > SW1D_EULER)
+        return fail(SW1D_EINVAL, "unknown equation %d", q->equation);
+    if (q->precision != SW1D_F64 && q->precision != SW1D_F32)
+        return fail(SW1D_EINVAL, "unknown precision %d", q->precision);
+    if (q->algorithm < SW1D_CLASSIC || q->algorithm > SW1D_TRAPEZOID)
+        return fail(SW1D_EINVAL, "unknown algorithm %d", q->algorithm);
+    if (q->nx < 8) return fail(SW1D_EINVAL, "nx must be >= 8 (got %lld)", (long long)q->nx);
+    if (!(q->dt > 0) || !std::isfinite(q->dt)) return fail(SW1D_EINVAL, "dt must be positive");
+
+    p->prob = *q;
+    p->eq = q->equation;
+    p->alg = q->algorithm;
+    p->f64 = q->precision == SW1D_F64;
+    p->n = q->nx;
+    p->S = p->f64 ? 8 : 4;
+    p->nc = eq_components(p->eq);
+    p->R = eq_radius(p->eq);
+    int bc = q->bc;
+    if (bc == SW1D_BC_DEFAULT) bc = p->eq == EQ_HEAT ? BC_MIRROR : (p->eq == EQ_KS ? BC_PERIODIC : BC_DIRICHLET);
+    if ((p->eq == EQ_HEAT && bc != BC_MIRROR && bc != BC_PERIODIC) ||
+        (p->eq == EQ_KS && bc != BC_PERIODIC) || (p->eq == EQ_EULER && bc != BC_DIRICHLET)) {
+        return fail(SW1D_EINVAL, "boundary condition %d is not defined for this equation", q->bc);
+    }
+    p->bc = bc;
+
+    double euler_dx = 0;
+    fill_coefs(p, &euler_dx);
+    if (p->eq == EQ_EULER && p->consts[2] > 0.21) {
+        return fail(SW1D_EUNSTABLE,
+                    "The value of dt/dx (%g) is too high.  In general it must be <=.21 for stability.",
+                    p->consts[2]);
+    }
+
+    // Reference cycle: t_fullstep / dt (Heat:483/541, KS:616/676/755, Euler:863/924).
+    const int tpb = q->tpb > 0 ? q->tpb : 64;
+    if (p->alg == SW1D_CLASSIC) p->cycle_steps = p->eq == EQ_HEAT ? 2 : 1;
+    else if (p->alg == SW1D_TRAPEZOID) p->cycle_steps = 0;
+    else if (p->eq == EQ_HEAT) p->cycle_steps = tpb;
+    else if (p->eq == EQ_KS && p->alg == SW1D_ALT) p->cycle_steps = 8;
+    else p->cycle_steps = tpb / 4;
+
+    if (p->alg == SW1D_TRAPEZOID) {
+        const int SB = q->tile_depth > 0 ? q->tile_depth : 1;
+        int W = q->node_width > 0 ? q->node_width
+              : (p->eq == EQ_HEAT ? 1024 : (p->eq == EQ_KS ? 512 : 256));
+        if (!is_pow2(W) || W < 128 || !trapezoid_supported(p->eq, W / 64, SB)) {
+            return fail(SW1D_ENOTSUP, "no trapezoid tile of %d points and depth %d for this equation",
+                        W, SB);
+        }
+        if (p->n < 2 * (int64_t)W) {
+            return fail(SW1D_ENOTSUP, "nx=%lld is below two trapezoid tiles of %d points",
+                        (long long)p->n, W);
+        }
+        p->W = W;
+        p->P = W / 64;
+        p->SB = SB;
+        p->tile_steps = trapezoid_steps(p->eq, SB);
+    } else if (p->alg != SW1D_CLASSIC) {
+        int W = q->node_width > 0 ? q->node_width : tpb;
+        if (p->eq == EQ_KS && p->alg == SW1D_ALT && q->node_width <= 0) W = 128;
+        const int pmin = min_points_per_lane(p->eq), pmax = max_points_per_lane(p->eq, p->f64);
+        if (!is_pow2(W)) {
+            return fail(SW1D_EINVAL, "swept node width must be a power of two (got %d)", W);
+        }
+        W = std::max(W, 64 * pmin);
+        W = std::min(W, 64 * pmax);
+        while (W > 64 * pmin && (p->n % W != 0 || p->n < 2 * (int64_t)W)) W /= 2;
+        if (p->n % W != 0 || p->n < 2 * (int64_t)W) {
+            return fail(SW1D_ENOTSUP, "nx=%lld cannot be split into swept nodes of >= %d points",
+                        (long long)p->n, 64 * pmin);
+        }
+        p->W = W;
+        p->P = W / 64;
+        p->tile_steps = p->eq == EQ_HEAT ? W / 2 : W / 8;
+    }
+    return SW1D_OK;
+}
+
+}  // namespace
+
+// ================================================ allocation-free advance ===
+// Workspace of sw1d_advance_noalloc: the two component-major state buffers of
+// the plan it stands in for, as file-scope device memory (one copy per GPU,
+// held until the process ends).  On the host `scratch` is only a stand-in
+// symbol, never a device address: host code uses the address that
+// hipGetSymbolAddress returns on the current device (g_scratch_ptr).
+static __device__ __align__(256) unsigned char scratch[SW1D_NOALLOC_MAX_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Once per device, with the device current: resolve the workspace's device
+// address and run the lane-shift check in its first word (synchronous, on
+// the null stream).
+int noalloc_device_init(int dev, void** ws)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* a = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&a, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !a)
+            return fail(SW1D_EDEVICE, "hipGetSymbolAddress(workspace) on device %d: %s", dev,
+                        hipGetErrorString(e));
+        if (g_lane_check == 0) {
+            const int r = check_lane_shifts_at(static_cast<int*>(a));
+            if (r < 0) return fail(SW1D_EDEVICE, "lane-shift check could not run on device %d", dev);
+            g_lane_check = r == 1 ? 1 : -1;
+        }
+        g_scratch_ptr[dev] = a;
+    }
+    *ws = g_scratch_ptr[dev];
+    return SW1D_OK;
+}
+
 }  // namespace
 
 // ================================================================= C ABI ===
@@ -643,92 +766,13 @@ int sw1d_plan_create(sw1d_plan** out, const sw1d_problem* q)
 {
     if (!out || !q) return fail(SW1D_EINVAL, "null argument");
     *out = nullptr;
-    if (q->equation < SW1D_HEAT || q->equation > SW1D_EULER)
-        return fail(SW1D_EINVAL, "unknown equation %d", q->equation);
-    if (q->precision != SW1D_F64 && q->precision != SW1D_F32)
-        return fail(SW1D_EINVAL, "unknown precision %d", q->precision);
-    if (q->algorithm < SW1D_CLASSIC || q->algorithm > SW1D_TRAPEZOID)
-        return fail(SW1D_EINVAL, "unknown algorithm %d", q->algorithm);
-    if (q->nx < 8) return fail(SW1D_EINVAL, "nx must be >= 8 (got %lld)", (long long)q->nx);
-    if (!(q->dt > 0) || !std::isfinite(q->dt)) return fail(SW1D_EINVAL, "dt must be positive");
-
     sw1d_plan* p = new sw1d_plan();
-    p->prob = *q;
-    p->eq = q->equation;
-    p->alg = q->algorithm;
-    p->f64 = q->precision == SW1D_F64;
-    p->n = q->nx;
-    p->S = p->f64 ? 8 : 4;
-    p->nc = eq_components(p->eq);
-    p->R = eq_radius(p->eq);
-    int bc = q->bc;
-    if (bc == SW1D_BC_DEFAULT) bc = p->eq == EQ_HEAT ? BC_MIRROR : (p->eq == EQ_KS ? BC_PERIODIC : BC_DIRICHLET);
-    if ((p->eq == EQ_HEAT && bc != BC_MIRROR && bc != BC_PERIODIC) ||
-        (p->eq == EQ_KS && bc != BC_PERIODIC) || (p->eq == EQ_EULER && bc != BC_DIRICHLET)) {
+    int rc = plan_setup(p, q);
+    if (rc) {
         delete p;
-        return fail(SW1D_EINVAL, "boundary condition %d is not defined for this equation", q->bc);
+        return rc;
     }
-    p->bc = bc;
-
-    double euler_dx = 0;
-    fill_coefs(p, &euler_dx);
-    if (p->eq == EQ_EULER && p->consts[2] > 0.21) {
-        delete p;
-        return fail(SW1D_EUNSTABLE,
-                    "The value of dt/dx (%g) is too high.  In general it must be <=.21 for stability.",
-                    p->consts[2]);
-    }
-
-    // Reference cycle: t_fullstep / dt (Heat:483/541, KS:616/676/755, Euler:863/924).
-    const int tpb = q->tpb > 0 ? q->tpb : 64;
-    if (p->alg == SW1D_CLASSIC) p->cycle_steps = p->eq == EQ_HEAT ? 2 : 1;
-    else if (p->alg == SW1D_TRAPEZOID) p->cycle_steps = 0;
-    else if (p->eq == EQ_HEAT) p->cycle_steps = tpb;
-    else if (p->eq == EQ_KS && p->alg == SW1D_ALT) p->cycle_steps = 8;
-    else p->cycle_steps = tpb / 4;
-
-    if (p->alg == SW1D_TRAPEZOID) {
-        const int SB = q->tile_depth > 0 ? q->tile_depth : 1;
-        int W = q->node_width > 0 ? q->node_width
-              : (p->eq == EQ_HEAT ? 1024 : (p->eq == EQ_KS ? 512 : 256));
-        if (!is_pow2(W) || W < 128 || !trapezoid_supported(p->eq, W / 64, SB)) {
-            delete p;
-            return fail(SW1D_ENOTSUP, "no trapezoid tile of %d points and depth %d for this equation",
-                        W, SB);
-        }
-        if (p->n < 2 * (int64_t)W) {
-            delete p;
-            return fail(SW1D_ENOTSUP, "nx=%lld is below two trapezoid tiles of %d points",
-                        (long long)p->n, W);
-        }
-        p->W = W;
-        p->P = W / 64;
-        p->SB = SB;
-        p->tile_steps = trapezoid_steps(p->eq, SB);
-        if (g_lane_check == 0) sw1d_device_check();
-        if (g_lane_check != 1) {
-            delete p;
-            return fail(SW1D_EDEVICE, "DPP lane-shift self check failed on this device");
-        }
-    } else if (p->alg != SW1D_CLASSIC) {
-        int W = q->node_width > 0 ? q->node_width : tpb;
-        if (p->eq == EQ_KS && p->alg == SW1D_ALT && q->node_width <= 0) W = 128;
-        const int pmin = min_points_per_lane(p->eq), pmax = max_points_per_lane(p->eq, p->f64);
-        if (!is_pow2(W)) {
-            delete p;
-            return fail(SW1D_EINVAL, "swept node width must be a power of two (got %d)", W);
-        }
-        W = std::max(W, 64 * pmin);
-        W = std::min(W, 64 * pmax);
-        while (W > 64 * pmin && (p->n % W != 0 || p->n < 2 * (int64_t)W)) W /= 2;
-        if (p->n % W != 0 || p->n < 2 * (int64_t)W) {
-            delete p;
-            return fail(SW1D_ENOTSUP, "nx=%lld cannot be split into swept nodes of >= %d points",
-                        (long long)p->n, 64 * pmin);
-        }
-        p->W = W;
-        p->P = W / 64;
-        p->tile_steps = p->eq == EQ_HEAT ? W / 2 : W / 8;
+    if (p->alg != SW1D_CLASSIC) {
         if (g_lane_check == 0) sw1d_device_check();
         if (g_lane_check != 1) {
             delete p;
@@ -779,7 +823,10 @@ int sw1d_plan_destroy(sw1d_plan* p)
     return SW1D_OK;
 }
 
-static int copy_state(sw1d_plan* p, const void* src, void* dst, bool to_device, hipMemcpyKind kind)
+// Copies the state in or out of the plan's current buffer on its stream; with
+// sync, waits for the copy.
+static int copy_state(sw1d_plan* p, const void* src, void* dst, bool to_device, hipMemcpyKind kind,
+                      bool sync = true)
 {
     const size_t S = p->S;
     const int64_t n = p->n;
@@ -811,7 +858,7 @@ static int copy_state(sw1d_plan* p, const void* src, void* dst, bool to_device, 
             if (host) HIP_OR_FAIL(hipMemcpyAsync(dst, scratch, 3 * n * S, kind, p->stream));
         }
     }
-    HIP_OR_FAIL(hipStreamSynchronize(p->stream));
+    if (sync) HIP_OR_FAIL(hipStreamSynchronize(p->stream));
     return SW1D_OK;
 }
 
@@ -850,6 +897,57 @@ int sw1d_advance(sw1d_plan* p, int64_t nsteps)
         rc = trapezoid_advance(p, nsteps);
     } else {
         rc = swept_advance(p, nsteps);
+    }
+    return rc;
+}
+
+int sw1d_advance_noalloc(const sw1d_problem* q, void* d_state, int64_t nsteps, void* stream)
+{
+    if (!q || !d_state) return fail(SW1D_EINVAL, "null argument");
+    if (nsteps < 0) return fail(SW1D_EINVAL, "nsteps must be >= 0");
+    sw1d_plan p;  // on the stack, no buffers of its own
+    int rc = plan_setup(&p, q);
+    if (rc) return rc;
+    if (p.alg == SW1D_ALT && p.eq != EQ_KS)
+        return fail(SW1D_ENOTSUP, "the hybrid CPU+GPU algorithm needs pinned host memory; "
+                                  "the allocation-free advance runs the GPU-only algorithms");
+    const size_t half = (size_t)p.nc * (size_t)p.n * p.S;
+    if (2 * half > (size_t)SW1D_NOALLOC_MAX_BYTES)
+        return fail(SW1D_EINVAL, "nx=%lld needs %zu bytes of workspace; the allocation-free advance "
+                    "holds %zu per GPU (use a plan)", (long long)p.n, 2 * half,
+                    (size_t)SW1D_NOALLOC_MAX_BYTES);
+    int dev = -1;
+    HIP_OR_FAIL(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(SW1D_EDEVICE, "device %d out of range", dev);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    void* ws = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        ws = g_scratch_ptr[dev];
+    }
+    if (!ws) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        HIP_OR_FAIL(hipStreamIsCapturing(s, &cs));
+        if (cs != hipStreamCaptureStatusNone)
+            return fail(SW1D_EINVAL, "the first allocation-free advance on device %d sets up the "
+                        "workspace synchronously and cannot run under stream capture", dev);
+        rc = noalloc_device_init(dev, &ws);
+        if (rc) return rc;
+    }
+    if (p.alg != SW1D_CLASSIC && g_lane_check != 1)
+        return fail(SW1D_EDEVICE, "DPP lane-shift self check failed on this device");
+    p.buf[0] = ws;
+    p.buf[1] = static_cast<unsigned char*>(ws) + half;
+    p.stream = s;
+    // launches one by one: no launch graph is captured or instantiated
+    p.classic_graph_failed = true;
+    p.tile_graph_failed = true;
+    rc = copy_state(&p, d_state, nullptr, true, hipMemcpyDeviceToDevice, false);
+    if (rc == SW1D_OK) rc = sw1d_advance(&p, nsteps);
+    if (rc == SW1D_OK) rc = copy_state(&p, nullptr, d_state, false, hipMemcpyDeviceToDevice, false);
+    if (rc == SW1D_OK) {
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) rc = fail(SW1D_EDEVICE, "launch: %s", hipGetErrorString(e));
     }
     return rc;
 }

@@ -106,6 +106,29 @@ int sw1d_synchronize(sw1d_plan* plan);
  * the plan's stream; replaces the cudaEvent pair of Heat:774-794). */
 int sw1d_advance_timed(sw1d_plan* plan, int64_t nsteps, double* elapsed_ms);
 
+/* Allocation-free advance -------------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  Advances a caller-owned device state in place by nsteps
+ * steps and gives the bits of sw1d_plan_create + sw1d_set_state_device +
+ * sw1d_advance + sw1d_get_state_device, without a plan: the two state
+ * buffers live in a library-owned static device workspace of
+ * SW1D_NOALLOC_MAX_BYTES on each GPU (held until the process ends), and the
+ * call makes no hipMalloc/hipFree, creates no stream, event or graph.
+ *   d_state  device pointer, layout of sw1d_set_state (read, then overwritten)
+ *   stream   hipStream_t as void* (NULL: the null stream).  The call only
+ *            enqueues work on it; the caller synchronises.  The first call on
+ *            a device resolves the workspace and runs the lane-shift check
+ *            synchronously, so it must not be under stream capture.
+ * Limit: 2 * values_per_point * nx * sizeof(REAL) <= SW1D_NOALLOC_MAX_BYTES
+ * (64 MiB: heat and KS FP64 up to 2^21 points, Euler FP64 up to 699050
+ * cells); larger problems return SW1D_EINVAL.  The hybrid CPU+GPU algorithm
+ * (heat/Euler SW1D_ALT) returns SW1D_ENOTSUP.  All calls on one GPU share its
+ * workspace, which keeps the last call's contents: calls that overlap on one
+ * GPU (other streams or host threads) race. */
+#define SW1D_NOALLOC_MAX_BYTES (64ull << 20)
+int sw1d_advance_noalloc(const sw1d_problem* problem, void* d_state, int64_t nsteps, void* stream);
+
 /* Plan facts ----------------------------------------------------------------
  * info[0] node width (points per swept node), info[1] steps advanced by one
  * swept tile launch, info[2] values stored per point (2 heat and KS, 6 Euler),
