@@ -34,6 +34,7 @@ struct SweptLaunch {
     int64_t n, nnodes, node_first, offset;
     const void* in;
     void* out;
+    int simds = 0;  // SIMDs of the device (0: unknown), which picks the stage form
 };
 
 struct Coefs {

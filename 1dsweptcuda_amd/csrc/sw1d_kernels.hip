@@ -97,7 +97,7 @@ __device__ __forceinline__ void for_each_index(F&& f, std::integer_sequence<int,
 // needs the level before.  Odd stages write row 1 from row 0, even stages
 // row 0 from row 1.  HeatRegion holds PC points per lane of the middle
 // 64*PC points of a node (all of it when PC = P).
-template <typename T, int PC, bool MIRROR>
+template <typename T, int PC, bool MIRROR, bool XM>
 struct HeatRegion {
     T r0[PC], r1[PC];
     T hl0[1], hr0[1], hl1[1], hr1[1];
@@ -117,8 +117,13 @@ struct HeatRegion {
         }
 #pragma unroll
         for (int p = 0; p < PC; ++p) {
-            const T nv = heat_ftcs(e[p], e[p + 1], e[p + 2], fo, fo2);
-            dst[p] = (d[p] <= thr) ? nv : dst[p];
+            if constexpr (XM) {
+                // heat_ftcs = fma(Fo, l + r, (1-2Fo) c), its fma in the active lanes
+                fma_where(dst[p], fo, e[p] + e[p + 2], fo2 * e[p + 1], d[p], thr);
+            } else {
+                const T nv = heat_ftcs(e[p], e[p + 1], e[p + 2], fo, fo2);
+                dst[p] = (d[p] <= thr) ? nv : dst[p];
+            }
         }
     }
     // down pairs s = s0, s0+2, .. s1 (thresholds 2s-1, 2s+1); up pairs
@@ -260,7 +265,7 @@ __device__ __forceinline__ void wave_sync()
 // active set and the one-point stencil reach:
 // the same operations, so the same bits, on fewer points.  The node moves
 // between HBM and the image ring by ring (NodeRings).
-template <typename T, int P, int TILE, bool MIRROR>
+template <typename T, int P, int TILE, bool MIRROR, bool XM>
 __global__ __launch_bounds__(kBlock) void k_heat_swept(const T* __restrict__ in,
                                                        T* __restrict__ out, int64_t n,
                                                        int64_t nnodes, int64_t node_first,
@@ -275,7 +280,7 @@ __global__ __launch_bounds__(kBlock) void k_heat_swept(const T* __restrict__ in,
     if (!node_geom<P>(n, nnodes, node_first, offset, g)) return;
 
     const T* __restrict__ in1 = in + n;
-    HeatRegion<T, P, MIRROR> full;
+    HeatRegion<T, P, MIRROR, XM> full;
     NodeRings<T, P, 2> rings(g, n);
     if constexpr (RING_IN) {
         rings.load_all(in, n);
@@ -316,7 +321,7 @@ __global__ __launch_bounds__(kBlock) void k_heat_swept(const T* __restrict__ in,
                 rings.template put<PC + 1>(img);  // region PC's halo
                 wave_sync();
             }
-            HeatRegion<T, PC, MIRROR> r;
+            HeatRegion<T, PC, MIRROR, XM> r;
             r.template load<W>(img, lane, g.base, n);
             r.down(32 * (PC - 1) + 1, 32 * PC - 1, fo, fo2);
             r.template save<W>(img, lane);
@@ -324,7 +329,7 @@ __global__ __launch_bounds__(kBlock) void k_heat_swept(const T* __restrict__ in,
         };
         auto up_region = [&](auto pc_tag) {
             constexpr int PC = decltype(pc_tag)::value;
-            HeatRegion<T, PC, MIRROR> r;
+            HeatRegion<T, PC, MIRROR, XM> r;
             r.template load<W>(img, lane, g.base, n);
             if constexpr (PC == 1) {
                 r.up(W, 32 * (P - PC) + 1, H - 3, fo, fo2);
@@ -369,7 +374,7 @@ __global__ __launch_bounds__(kBlock) void k_heat_swept(const T* __restrict__ in,
 // halos in lane 0 / lane 63 of the carried neighbour registers); d[p] is the point's distance code |2x - (W-1)| from
 // the node centre, and a stage with threshold thr updates the points with
 // d <= thr (the diamond's active set) and keeps the others.
-template <typename T, int PC>
+template <typename T, int PC, bool XM>
 struct KsRegion {
     T u[PC], uh[PC];
     T hlu[2], hru[2], hlh[2], hrh[2];
@@ -382,8 +387,14 @@ struct KsRegion {
         gather_carry<T, PC, 2>(u, hlu, hru, e);
 #pragma unroll
         for (int p = 0; p < PC; ++p) {
-            const T nv = ks_predict(e[p], e[p + 1], e[p + 2], e[p + 3], e[p + 4], k);
-            uh[p] = (d[p] <= thr) ? nv : uh[p];
+            if constexpr (XM) {
+                // ks_predict = fma(-dt/2, F, c), its fma in the active lanes
+                fma_where(uh[p], -k.dth, ks_rhs(e[p], e[p + 1], e[p + 2], e[p + 3], e[p + 4], k), u[p], d[p],
+                          thr);
+            } else {
+                const T nv = ks_predict(e[p], e[p + 1], e[p + 2], e[p + 3], e[p + 4], k);
+                uh[p] = (d[p] <= thr) ? nv : uh[p];
+            }
         }
     }
     // Corrector stage (KS:166-171, :202): u += -dt F(u_half).
@@ -393,8 +404,14 @@ struct KsRegion {
         gather_carry<T, PC, 2>(uh, hlh, hrh, e);
 #pragma unroll
         for (int p = 0; p < PC; ++p) {
-            const T nv = ks_correct(u[p], e[p], e[p + 1], e[p + 2], e[p + 3], e[p + 4], k);
-            u[p] = (d[p] <= thr) ? nv : u[p];
+            if constexpr (XM) {
+                // ks_correct = fma(-dt, F, u), its fma in the active lanes
+                fma_where(u[p], -k.dt, ks_rhs(e[p], e[p + 1], e[p + 2], e[p + 3], e[p + 4], k), u[p], d[p],
+                          thr);
+            } else {
+                const T nv = ks_correct(u[p], e[p], e[p + 1], e[p + 2], e[p + 3], e[p + 4], k);
+                u[p] = (d[p] <= thr) ? nv : u[p];
+            }
         }
     }
     // Down-triangle pairs s = s0, s0+2, .. s1 (thresholds 4s-1, 4s+3) and
@@ -451,7 +468,7 @@ struct KsRegion {
 // one-width form, so the same bits, with 46% fewer point evaluations at
 // P = 16 (42% at P = 8, 31% at P = 4).  The node moves between HBM and the
 // image ring by ring (NodeRings).
-template <typename T, int P, int TILE>
+template <typename T, int P, int TILE, bool XM>
 __global__ __launch_bounds__(kBlock) void k_ks_swept(const T* __restrict__ in,
                                                      T* __restrict__ out, int64_t n,
                                                      int64_t nnodes, int64_t node_first,
@@ -466,7 +483,7 @@ __global__ __launch_bounds__(kBlock) void k_ks_swept(const T* __restrict__ in,
     if (!node_geom<P>(n, nnodes, node_first, offset, g)) return;
 
     const T* __restrict__ in_h = in + n;
-    KsRegion<T, P> full;
+    KsRegion<T, P, XM> full;
     NodeRings<T, P, 2> rings(g, n);
     if constexpr (RING_IN) {
         rings.load_all(in, n);
@@ -513,7 +530,7 @@ __global__ __launch_bounds__(kBlock) void k_ks_swept(const T* __restrict__ in,
                 rings.template put<PC + 1>(img);
                 wave_sync();
             }
-            KsRegion<T, PC> r;
+            KsRegion<T, PC, XM> r;
             r.template load<W>(img, lane);
             r.down(PC == 2 ? 1 : 16 * PC - 15, 16 * PC - 1, k);
             r.template save<W>(img, lane);
@@ -521,7 +538,7 @@ __global__ __launch_bounds__(kBlock) void k_ks_swept(const T* __restrict__ in,
         };
         auto up_region = [&](auto pc_tag) {
             constexpr int PC = decltype(pc_tag)::value;
-            KsRegion<T, PC> r;
+            KsRegion<T, PC, XM> r;
             r.template load<W>(img, lane);
             if constexpr (PC == 2) {
                 r.up(W, 16 * (P - 2) + 1, H - 3, k);
@@ -913,16 +930,34 @@ hipError_t swept_tile(const SweptLaunch& L, const Coefs& c, hipStream_t s)
     const dim3 grid((unsigned)((count + kWavesPerBlock - 1) / kWavesPerBlock));
     const T* in = static_cast<const T*>(L.in);
     T* out = static_cast<T*>(L.out);
+    // Two or more nodes (waves) per SIMD: the stages' final fma runs under an
+    // EXEC mask (v_cmpx), and the other wave issues while one runs its scalar
+    // EXEC restore.  One wave per SIMD: the compare-and-select form, which
+    // needs no scalar instruction between the FP64 work.  Same bits.
+    const bool xm = L.simds > 0 && count >= 2 * (int64_t)L.simds;
     if (L.eq == EQ_HEAT) {
-        if (L.bc == BC_PERIODIC)
-            hipLaunchKernelGGL((k_heat_swept<T, P, TILE, false>), grid, dim3(kBlock), 0, s, in, out,
-                               L.n, L.nnodes, L.node_first, L.offset, fo_of<T>(c), fo2_of<T>(c));
-        else
-            hipLaunchKernelGGL((k_heat_swept<T, P, TILE, true>), grid, dim3(kBlock), 0, s, in, out,
-                               L.n, L.nnodes, L.node_first, L.offset, fo_of<T>(c), fo2_of<T>(c));
+        if (L.bc == BC_PERIODIC) {
+            if (xm)
+                hipLaunchKernelGGL((k_heat_swept<T, P, TILE, false, true>), grid, dim3(kBlock), 0, s, in, out,
+                                   L.n, L.nnodes, L.node_first, L.offset, fo_of<T>(c), fo2_of<T>(c));
+            else
+                hipLaunchKernelGGL((k_heat_swept<T, P, TILE, false, false>), grid, dim3(kBlock), 0, s, in, out,
+                                   L.n, L.nnodes, L.node_first, L.offset, fo_of<T>(c), fo2_of<T>(c));
+        } else {
+            if (xm)
+                hipLaunchKernelGGL((k_heat_swept<T, P, TILE, true, true>), grid, dim3(kBlock), 0, s, in, out,
+                                   L.n, L.nnodes, L.node_first, L.offset, fo_of<T>(c), fo2_of<T>(c));
+            else
+                hipLaunchKernelGGL((k_heat_swept<T, P, TILE, true, false>), grid, dim3(kBlock), 0, s, in, out,
+                                   L.n, L.nnodes, L.node_first, L.offset, fo_of<T>(c), fo2_of<T>(c));
+        }
     } else if (L.eq == EQ_KS) {
-        hipLaunchKernelGGL((k_ks_swept<T, P, TILE>), grid, dim3(kBlock), 0, s, in, out, L.n,
-                           L.nnodes, L.node_first, L.offset, ks_of<T>(c));
+        if (xm)
+            hipLaunchKernelGGL((k_ks_swept<T, P, TILE, true>), grid, dim3(kBlock), 0, s, in, out, L.n,
+                               L.nnodes, L.node_first, L.offset, ks_of<T>(c));
+        else
+            hipLaunchKernelGGL((k_ks_swept<T, P, TILE, false>), grid, dim3(kBlock), 0, s, in, out, L.n,
+                               L.nnodes, L.node_first, L.offset, ks_of<T>(c));
     } else {
         if constexpr (P <= 8)
             hipLaunchKernelGGL((k_euler_swept<T, P, TILE>), grid, dim3(kBlock), 0, s, in, out,

@@ -77,5 +77,28 @@ __device__ __forceinline__ void gather_carry(const T (&v)[P], T (&nl)[R], T (&nr
     }
 }
 
+// dst = fma(a, b, c), one rounding, in the lanes where d <= thr; the other
+// lanes keep dst.  v_cmpx narrows EXEC to those lanes for the one FMA, and
+// EXEC is then set back to every lane: the tile kernels that call this run
+// whole waves with every lane active (one node per wave).  Two issue slots
+// instead of a compare and two 32-bit selects.
+__device__ __forceinline__ void fma_where(double& dst, double a, double b, double c, int d, int thr)
+{
+    asm volatile("v_cmpx_ge_i32_e32 vcc, %4, %5\n\t"
+                 "v_fma_f64 %0, %1, %2, %3\n\t"
+                 "s_mov_b64 exec, -1"
+                 : "+v"(dst)
+                 : "v"(a), "v"(b), "v"(c), "s"(thr), "v"(d)
+                 : "vcc");
+}
+__device__ __forceinline__ void fma_where(float& dst, float a, float b, float c, int d, int thr)
+{
+    asm volatile("v_cmpx_ge_i32_e32 vcc, %4, %5\n\t"
+                 "v_fma_f32 %0, %1, %2, %3\n\t"
+                 "s_mov_b64 exec, -1"
+                 : "+v"(dst)
+                 : "v"(a), "v"(b), "v"(c), "s"(thr), "v"(d)
+                 : "vcc");
+}
 
 }  // namespace sw1d

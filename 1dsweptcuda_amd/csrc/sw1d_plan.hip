@@ -4,9 +4,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -55,6 +57,7 @@ struct sw1d_plan {
     int SB = 0;        // trapezoid: extra points per lane (stages per launch / (32/R))
     int R = 1;         // stage radius
     int64_t tile_steps = 0;  // steps advanced by one swept tile launch
+    int simds = 0;           // SIMDs of the plan's device (4 per compute unit)
     int64_t cycle_steps = 0; // reference t_fullstep / dt
     Coefs coef{};
     double consts[9]{};
@@ -176,6 +179,19 @@ void fill_coefs(sw1d_plan* p, double* euler_dx)
 
 bool is_pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
 
+// SW1D_LAUNCH_GRAPHS=0 in the environment: plans and batches launch their
+// kernels one by one instead of replaying hipGraphs (the same kernels in the
+// same order, so the same bits).  For tools that do not follow graph launches,
+// such as hardware-counter collection.
+bool launch_graphs_enabled()
+{
+    static const bool on = [] {
+        const char* v = std::getenv("SW1D_LAUNCH_GRAPHS");
+        return !(v && v[0] == '0');
+    }();
+    return on;
+}
+
 // splitmix64 (Steele, Lea, Flood 2014): a portable generator, so the same seed
 // gives the same synthetic initial condition on every host.
 uint64_t splitmix64(uint64_t& s)
@@ -194,6 +210,22 @@ double splitmix_normal(uint64_t& s)  // Box-Muller, cosine branch
     const double u1 = 1.0 - splitmix_uniform(s);
     const double u2 = splitmix_uniform(s);
     return sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
+}
+
+// SIMDs of the current device: 4 per compute unit on CDNA (0 if the query
+// fails).  Cached per device; a plain attribute query, no allocation.
+int device_simds()
+{
+    static std::atomic<int> cache[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+    int v = cache[dev].load(std::memory_order_relaxed);
+    if (v > 0) return v;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        return 0;
+    cache[dev].store(4 * cus, std::memory_order_relaxed);
+    return 4 * cus;
 }
 
 // ------------------------------------------------------------ profiling ---
@@ -317,6 +349,7 @@ int swept_launch(sw1d_plan* p, int tile, bool shifted, int64_t node_first)
     L.offset = shifted ? p->n - p->W / 2 : 0;
     L.in = p->buf[p->cur];
     L.out = p->buf[p->cur ^ 1];
+    L.simds = p->simds;
     const int kind = tile == TILE_UP ? KK_UP : tile == TILE_DOWN ? KK_DOWN
                    : (shifted ? KK_SPLIT : KK_DIAMOND);
     {
@@ -772,6 +805,8 @@ int sw1d_plan_create(sw1d_plan** out, const sw1d_problem* q)
         delete p;
         return rc;
     }
+    if (!launch_graphs_enabled()) p->classic_graph_failed = p->tile_graph_failed = true;
+    p->simds = device_simds();
     if (p->alg != SW1D_CLASSIC) {
         if (g_lane_check == 0) sw1d_device_check();
         if (g_lane_check != 1) {
@@ -939,6 +974,7 @@ int sw1d_advance_noalloc(const sw1d_problem* q, void* d_state, int64_t nsteps, v
     p.buf[0] = ws;
     p.buf[1] = static_cast<unsigned char*>(ws) + half;
     p.stream = s;
+    p.simds = device_simds();
     // launches one by one: no launch graph is captured or instantiated
     p.classic_graph_failed = true;
     p.tile_graph_failed = true;
@@ -1247,18 +1283,26 @@ int sw1d_batch_advance(sw1d_batch* B, int64_t nsteps)
     if (!B) return fail(SW1D_EINVAL, "null batch");
     if (nsteps < 0) return fail(SW1D_EINVAL, "nsteps must be >= 0");
     if (nsteps == 0) return SW1D_OK;
+    BatchLaunch L;
+    L.jobs = B->jobs;
+    L.njobs = (int32_t)B->count;
+    L.next = B->next;
+    L.a = B->a;
+    L.b = B->b;
+    L.nsteps = nsteps;
+    L.grid = B->grid;
+    L.waves = B->waves;
+    if (!launch_graphs_enabled()) {
+        HIP_OR_FAIL(launch_batch(L, B->f64, B->stream));
+        return SW1D_OK;
+    }
     if (B->graph_steps != nsteps) {
+        // the previous graph's launch may still run on the stream: let it
+        // finish before its executable is destroyed
+        HIP_OR_FAIL(hipStreamSynchronize(B->stream));
         if (B->exec) { (void)hipGraphExecDestroy(B->exec); B->exec = nullptr; }
         if (B->graph) { (void)hipGraphDestroy(B->graph); B->graph = nullptr; }
-        BatchLaunch L;
-        L.jobs = B->jobs;
-        L.njobs = (int32_t)B->count;
-        L.next = B->next;
-        L.a = B->a;
-        L.b = B->b;
-        L.nsteps = nsteps;
-        L.grid = B->grid;
-        L.waves = B->waves;
+        B->graph_steps = -1;
         HIP_OR_FAIL(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
         const hipError_t le = launch_batch(L, B->f64, B->stream);
         const hipError_t ce = hipStreamEndCapture(B->stream, &B->graph);

@@ -251,7 +251,7 @@ def _equations(sw, n):
     q[n // 2:] = (0.125, 0.0, 0.1 / 0.4)
     dt_eu = 0.1 / (n - 2)
     cases = {
-        "heat": (sw.HEAT, 1e-3, heat_ic, 20000, ((sw.CLASSIC, 256, 0), (sw.SWEPT, 1024, 0), (sw.ALT, 1024, 0),
+        "heat": (sw.HEAT, 1e-3, heat_ic, 20000, ((sw.CLASSIC, 256, 0), (sw.SWEPT, 512, 0), (sw.ALT, 512, 0),
                                                  (sw.TRAPEZOID, 256, 1))),
         "euler": (sw.EULER, dt_eu, q.ravel(), 2000, ((sw.CLASSIC, 256, 0), (sw.SWEPT, 512, 0), (sw.ALT, 512, 0),
                                                      (sw.TRAPEZOID, 256, 1))),
@@ -391,6 +391,8 @@ def _batched(sw, count=4096, steps=5000, loop_sample=512):
     for i in idx:
         p = sw.Plan(int(eq[i]), int(nx[i]), float(dt[i]), algorithm=sw.TRAPEZOID, node_width=256,
                     bc=sw.BC_PERIODIC, dx=float(dx[i]))
+        p.set_state(ics[i])
+        _warm(p, 0)  # captures the plan's launch graphs outside the timed loop
         p.set_state(ics[i])
         plans.append(p)
     for p in plans:

@@ -146,3 +146,53 @@ def test_batch_workload_full_shape(sw, orc, wl, c4_workload, prec):
         wants = list(pool.map(oracle, sample))
     bad = [(i, int(eq[i]), int(nx[i])) for i, w in zip(sample, wants) if not np.array_equal(got[i], w)]
     assert not bad, bad
+
+
+def test_direct_launches_without_graphs(sw, tmp_path):
+    """SW1D_LAUNCH_GRAPHS=0 (read once per process, so run in a child process):
+    trapezoid plans, classic plans and batches launch their kernels one by one
+    instead of replaying hipGraphs, with the same bits."""
+    import os
+    import subprocess
+    import sys
+
+    from conftest import ORACLE_DIR, ROOT
+
+    script = r'''
+import importlib, sys
+import numpy as np
+sys.path[:0] = [sys.argv[1], sys.argv[2]]
+import pyoracle as orc
+sw = importlib.import_module("1dsweptcuda_amd")
+wl = importlib.import_module("1dsweptcuda_amd.workloads")
+n, dt = 4096, 0.005
+u0 = orc.ic_ks_noisy(n, 3)
+want = orc.ks_run(u0, 700, orc.ks_consts(dt))
+for alg, kw in ((sw.TRAPEZOID, dict(node_width=512)), (sw.CLASSIC, {})):
+    with sw.Plan(sw.KS, n, dt, algorithm=alg, **kw) as p:
+        p.set_state(u0)
+        p.advance(300)
+        p.advance(400)
+        assert np.array_equal(p.get_state(), want), alg
+eq, nx, dts, dx = wl.batch_problems(24, seed=4, lo=8.0, hi=12.0)
+ics = wl.batch_ics(eq, nx, dx)
+with sw.Batch(eq, nx, dts, dx) as b:
+    b.set_state(np.concatenate(ics))
+    b.advance(64)
+    b.advance(100)
+    got = b.split(b.get_state())
+for i in range(24):
+    if eq[i] == 0:
+        fo, fo2 = orc.heat_consts(dts[i], ds=dx[i])
+        w = orc.heat_run(ics[i], 164, fo, fo2, 1)
+    else:
+        w = orc.ks_run(ics[i], 164, orc.ks_consts(dts[i], dx=dx[i]))
+    assert np.array_equal(got[i], w), i
+print("direct launches ok")
+'''
+    print("direct launches (SW1D_LAUNCH_GRAPHS=0) in a child process", flush=True)
+    env = dict(os.environ, SW1D_LAUNCH_GRAPHS="0")
+    r = subprocess.run([sys.executable, "-c", script, ROOT, ORACLE_DIR], env=env, capture_output=True,
+                       text=True, timeout=240)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "direct launches ok" in r.stdout

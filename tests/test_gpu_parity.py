@@ -256,3 +256,34 @@ def test_periodic_heat_configuration_c1(sw, orc, alg):
         p.advance(2000)
         got = p.get_state()
     assert np.array_equal(got, want)
+
+
+# (equation, bc, precision, algorithm, node width, log2 nX): at least 2048
+# nodes, two per SIMD of MI355X's 1024, where the swept stages run their final
+# fma under an EXEC mask instead of a compare and select
+XM_CASES = [(0, 0, 0, 1, 128, 18), (0, 2, 1, 1, 64, 17), (0, 0, 0, 2, 128, 18), (1, 0, 0, 1, 64, 17),
+            (1, 0, 1, 1, 256, 19), (1, 0, 0, 1, 512, 20), (1, 0, 0, 2, 128, 18), (1, 0, 1, 2, 128, 18)]
+
+
+@pytest.mark.parametrize("eq,bc,prec,alg,W,lg", XM_CASES)
+def test_exec_masked_stage_form(sw, orc, eq, bc, prec, alg, W, lg):
+    """Plans with two or more nodes per SIMD take the EXEC-masked form of the
+    swept stages; it gives the oracle's bits like the compare-and-select form."""
+    n = 1 << lg
+    dtype = np.float64 if prec == 0 else np.float32
+    dt = _dt(eq, n, bc)
+    H = W // 2 if eq == 0 else W // 8
+    nsteps = 3 * H + 5
+    _say(f"exec-masked stages eq={eq} bc={bc} prec={prec} alg={alg} W={W} n={n} steps={nsteps}")
+    u0 = _ic(orc, eq, n, dtype, bc)
+    with sw.Plan(eq, n, dt, precision=prec, algorithm=alg, tpb=W, node_width=W, bc=bc) as p:
+        assert p.info()["node_width"] == W and n // W >= 2048
+        p.set_state(u0)
+        p.advance(nsteps)
+        got = p.get_state()
+    if eq == 0:
+        fo, fo2 = orc.heat_consts(dt, dtype)
+        want = orc.heat_run(u0, nsteps, fo, fo2, 0 if bc != 2 else 1, 8)
+    else:
+        want = orc.ks_run(u0, nsteps, orc.ks_consts(dt, dtype), 8)
+    assert np.array_equal(got, want)
