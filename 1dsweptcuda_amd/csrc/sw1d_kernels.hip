@@ -930,11 +930,13 @@ hipError_t swept_tile(const SweptLaunch& L, const Coefs& c, hipStream_t s)
     const dim3 grid((unsigned)((count + kWavesPerBlock - 1) / kWavesPerBlock));
     const T* in = static_cast<const T*>(L.in);
     T* out = static_cast<T*>(L.out);
-    // Two or more nodes (waves) per SIMD: the stages' final fma runs under an
-    // EXEC mask (v_cmpx), and the other wave issues while one runs its scalar
-    // EXEC restore.  One wave per SIMD: the compare-and-select form, which
-    // needs no scalar instruction between the FP64 work.  Same bits.
-    const bool xm = L.simds > 0 && count >= 2 * (int64_t)L.simds;
+    // About two or more nodes (waves) per SIMD: the stages' final fma runs
+    // under an EXEC mask (v_cmpx), and another wave issues while one runs its
+    // scalar EXEC restore.  About one wave per SIMD: the compare-and-select
+    // form, which needs no scalar instruction between the FP64 work.  The
+    // threshold, 1.5 nodes per SIMD, puts the hybrid's nodes 1..K-1 with the
+    // full launch.  Same bits.
+    const bool xm = L.simds > 0 && 2 * count >= 3 * (int64_t)L.simds;
     if (L.eq == EQ_HEAT) {
         if (L.bc == BC_PERIODIC) {
             if (xm)
