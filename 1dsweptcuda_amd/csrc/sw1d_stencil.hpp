@@ -36,12 +36,37 @@ SW1D_HD T heat_ftcs(T l, T c, T r, T fo, T fo2)
 // CPU oracle uses the same ones): twelve FP operations per stage.
 template <typename T> struct KsCoef { T dx4x, dx2, dx4, dt, dth; };
 
+// fma(-4, l, ll) into a new register.  On the GPU the compiler would form it
+// as the two-address v_fmac and copy ll first (ll is still live as another
+// point's neighbour); the three-address v_fma_f64 / v_fma_f32 needs no copy.
+// The same single-rounding fused operation either way.
+SW1D_HD double fma_m4(double l, double ll)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    double t;
+    asm("v_fma_f64 %0, -4.0, %1, %2" : "=v"(t) : "v"(l), "v"(ll));
+    return t;
+#else
+    return fmadd(-4.0, l, ll);
+#endif
+}
+SW1D_HD float fma_m4(float l, float ll)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    float t;
+    asm("v_fma_f32 %0, -4.0, %1, %2" : "=v"(t) : "v"(l), "v"(ll));
+    return t;
+#else
+    return fmadd(-4.0f, l, ll);
+#endif
+}
+
 template <typename T>
 SW1D_HD T ks_rhs(T ll, T l, T c, T r, T rr, const KsCoef<T>& k)
 {
     const T conv = fmadd(r, r, -(l * l));                                   // r^2 - l^2
     const T sec = fmadd(T(-2), c, l + r);                                   // l + r - 2c
-    const T fourth = fmadd(T(-4), r, fmadd(T(6), c, fmadd(T(-4), l, ll))) + rr;
+    const T fourth = fmadd(T(-4), r, fmadd(T(6), c, fma_m4(l, ll))) + rr;
     return fmadd(conv, k.dx4x, fmadd(sec, k.dx2, fourth * k.dx4));
 }
 // KS:159-164 (stutterStep): predictor u_half = u - dt/2 * F(u).
