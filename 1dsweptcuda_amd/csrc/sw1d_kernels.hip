@@ -103,15 +103,18 @@ struct HeatRegion {
     T hl0[1], hr0[1], hl1[1], hr1[1];
     int d[PC];
     bool g0, gN;  // this lane's first / last point is global point 0 / n-1
+    bool bnd;     // some lane of the wave has g0 or gN (the same in every lane)
 
-    __device__ __forceinline__ void stage(int thr, const T (&src)[PC], T (&hl)[1], T (&hr)[1],
-                                          T (&dst)[PC], T fo, T fo2)
+    // MB: this wave holds point 0 or n-1 and applies the insulated ends.
+    template <bool MB>
+    __device__ __forceinline__ void stage_bc(int thr, const T (&src)[PC], T (&hl)[1], T (&hr)[1],
+                                             T (&dst)[PC], T fo, T fo2)
     {
         T e[PC + 2];
         gather_carry<T, PC, 1>(src, hl, hr, e);
         // Insulated ends (Heat:140-142): the left neighbour of point 0 is
         // point 1, the right neighbour of point n-1 is point n-2.
-        if (MIRROR) {
+        if constexpr (MB) {
             e[0] = g0 ? e[2] : e[0];
             e[PC + 1] = gN ? e[PC - 1] : e[PC + 1];
         }
@@ -126,21 +129,47 @@ struct HeatRegion {
             }
         }
     }
+    // One stage; the level loops below are unswitched on bnd, so the nodes
+    // without a domain end (all but one or two) run no boundary select.
+    __device__ __forceinline__ void stage(int thr, const T (&src)[PC], T (&hl)[1], T (&hr)[1],
+                                          T (&dst)[PC], T fo, T fo2)
+    {
+        if (MIRROR && bnd)
+            stage_bc<MIRROR>(thr, src, hl, hr, dst, fo, fo2);
+        else
+            stage_bc<false>(thr, src, hl, hr, dst, fo, fo2);
+    }
     // down pairs s = s0, s0+2, .. s1 (thresholds 2s-1, 2s+1); up pairs
     // m = m0, .. m1 (thresholds W-1-2m, W-3-2m)
-    __device__ __forceinline__ void down(int s0, int s1, T fo, T fo2)
+    template <bool MB>
+    __device__ __forceinline__ void down_bc(int s0, int s1, T fo, T fo2)
     {
         for (int s = s0; s <= s1; s += 2) {
-            stage(2 * s - 1, r0, hl0, hr0, r1, fo, fo2);
-            stage(2 * s + 1, r1, hl1, hr1, r0, fo, fo2);
+            stage_bc<MB>(2 * s - 1, r0, hl0, hr0, r1, fo, fo2);
+            stage_bc<MB>(2 * s + 1, r1, hl1, hr1, r0, fo, fo2);
         }
+    }
+    template <bool MB>
+    __device__ __forceinline__ void up_bc(int W, int m0, int m1, T fo, T fo2)
+    {
+        for (int m = m0; m <= m1; m += 2) {
+            stage_bc<MB>(W - 1 - 2 * m, r0, hl0, hr0, r1, fo, fo2);
+            stage_bc<MB>(W - 3 - 2 * m, r1, hl1, hr1, r0, fo, fo2);
+        }
+    }
+    __device__ __forceinline__ void down(int s0, int s1, T fo, T fo2)
+    {
+        if (MIRROR && bnd)
+            down_bc<MIRROR>(s0, s1, fo, fo2);
+        else
+            down_bc<false>(s0, s1, fo, fo2);
     }
     __device__ __forceinline__ void up(int W, int m0, int m1, T fo, T fo2)
     {
-        for (int m = m0; m <= m1; m += 2) {
-            stage(W - 1 - 2 * m, r0, hl0, hr0, r1, fo, fo2);
-            stage(W - 3 - 2 * m, r1, hl1, hr1, r0, fo, fo2);
-        }
+        if (MIRROR && bnd)
+            up_bc<MIRROR>(W, m0, m1, fo, fo2);
+        else
+            up_bc<false>(W, m0, m1, fo, fo2);
     }
     // flags and distance codes for a lane whose first point is node-local x0
     __device__ __forceinline__ void place(int W, int x0, int64_t base, int64_t n)
@@ -153,6 +182,7 @@ struct HeatRegion {
         gl = gl >= n ? gl - n : gl;
         g0 = MIRROR && gi == 0;
         gN = MIRROR && gl == n - 1;
+        bnd = MIRROR && __builtin_amdgcn_ballot_w64(g0 || gN) != 0;
     }
     template <int W>
     __device__ __forceinline__ void load(const T* img, int lane, int64_t base, int64_t n)
