@@ -90,6 +90,16 @@ __device__ __forceinline__ void for_each_index(F&& f, std::integer_sequence<int,
     (f(std::integral_constant<int, I>{}), ...);
 }
 
+// Calls f(integral_constant<int, j>) for j = A, A + 1, .. B in order (none if A > B).
+template <int A, int B, typename F>
+__device__ __forceinline__ void for_range(F&& f)
+{
+    if constexpr (A <= B) {
+        f(std::integral_constant<int, A>{});
+        for_range<A + 1, B>(f);
+    }
+}
+
 // ----------------------------------------------------------- heat tiles ---
 // Two values per point, the even-level and the odd-level row (the two rows of
 // the reference's working array, Heat:151-164): a point just inside a
@@ -489,14 +499,29 @@ struct KsRegion {
     }
 };
 
+// Region sizes of the phased KS tiles (points per lane): R_i = R_0 + i
+// kKsRegionStep, i = 0 .. ks_regions<P>() - 1, all below P, from R_0 =
+// kKsRegionFirst (P - 1 if that is not below P).
+constexpr int kKsRegionFirst = 1, kKsRegionStep = 1;
+template <int P>
+__host__ __device__ constexpr int ks_region(int i)
+{
+    return (kKsRegionFirst < P - 1 ? kKsRegionFirst : P - 1) + i * kKsRegionStep;
+}
+template <int P>
+__host__ __device__ constexpr int ks_regions()
+{
+    return (P - 1 - ks_region<P>(0)) / kKsRegionStep + 1;
+}
+
 // Swept KS tiles: one wave per node of W = 64*P points, lane l owning points
 // l*P .. l*P+P-1 in VGPRs.  A whole diamond runs W/2 stage levels but its
 // active set is W wide only in the middle; for P >= 4 the node's image is
-// kept in LDS and the levels run on centred regions of 2, 3, .. P points per
+// kept in LDS and the levels run on centred regions of 1, 2, .. P points per
 // lane, each level on the narrowest region that holds its active set and the
 // two-point stencil reach.  Same operations on the same values as the
-// one-width form, so the same bits, with 46% fewer point evaluations at
-// P = 16 (42% at P = 8, 31% at P = 4).  The node moves between HBM and the
+// one-width form, so the same bits, with 47% fewer point evaluations at
+// P = 16 (44% at P = 8, 38% at P = 4).  The node moves between HBM and the
 // image ring by ring (NodeRings).
 template <typename T, int P, int TILE, bool XM>
 __global__ __launch_bounds__(kBlock) void k_ks_swept(const T* __restrict__ in,
@@ -537,10 +562,14 @@ __global__ __launch_bounds__(kBlock) void k_ks_swept(const T* __restrict__ in,
             full.stage_a(W - 1 - 4 * (H - 1), k);
         }
     } else {
-        // Regions of PC = 2, 3, .. P-1 points per lane (widths 64 PC): region
-        // PC takes down pairs 16 PC - 15 <= s <= 16 PC - 1 (from 1 for PC = 2)
-        // and up pairs 16(P-PC)+1 <= m <= 16(P-PC)+15 (to H-3 for PC = 2),
-        // where every threshold is <= 64 PC - 1.
+        // Regions R_1 < R_2 < .. < R_K < P points per lane (widths 64 R_i,
+        // ks_region): region R_i takes down pairs 16 R_(i-1) + 1 <= s <=
+        // 16 R_i - 1 (from 1 for i = 1) and up pairs 16(P - R_i) + 1 <= m <=
+        // 16(P - R_(i-1)) - 1 (to H - 3 for i = 1), where every threshold is
+        // <= 64 R_i - 1; the full node takes the pairs above 16 R_K.
+        constexpr int K = ks_regions<P>();
+        static_assert(K >= 1, "phased KS tiles have at least one region");
+        constexpr int RK = ks_region<P>(K - 1);
         T* img = img_all + (threadIdx.x >> 6) * 2 * W;
         const int lane = g.lane;
         auto reload_full = [&] {
@@ -550,56 +579,57 @@ __global__ __launch_bounds__(kBlock) void k_ks_swept(const T* __restrict__ in,
                 full.uh[p] = img[W + lane * P + p];
             }
         };
-        auto down_region = [&](auto pc_tag) {
-            constexpr int PC = decltype(pc_tag)::value;
+        auto down_region = [&](auto i_tag) {
+            constexpr int I = decltype(i_tag)::value;
+            constexpr int PC = ks_region<P>(I), PREV = I == 0 ? 0 : ks_region<P>(I - 1);
             if constexpr (RING_IN) {
-                if constexpr (PC == 2) {
-                    rings.template put<1>(img);
-                    rings.template put<2>(img);
-                }
-                rings.template put<PC + 1>(img);
+                // rings 1 .. PC + 1 (PC + 1: the region's halo) are in the image
+                for_range<PREV + 2 - (I == 0 ? 1 : 0), PC + 1>(
+                    [&](auto j) { rings.template put<decltype(j)::value>(img); });
                 wave_sync();
             }
             KsRegion<T, PC, XM> r;
             r.template load<W>(img, lane);
-            r.down(PC == 2 ? 1 : 16 * PC - 15, 16 * PC - 1, k);
+            r.down(I == 0 ? 1 : 16 * PREV + 1, 16 * PC - 1, k);
             r.template save<W>(img, lane);
             wave_sync();
         };
-        auto up_region = [&](auto pc_tag) {
-            constexpr int PC = decltype(pc_tag)::value;
+        auto up_region = [&](auto j_tag) {
+            constexpr int I = K - 1 - decltype(j_tag)::value;
+            constexpr int PC = ks_region<P>(I), PREV = I == 0 ? 0 : ks_region<P>(I - 1);
             KsRegion<T, PC, XM> r;
             r.template load<W>(img, lane);
-            if constexpr (PC == 2) {
-                r.up(W, 16 * (P - 2) + 1, H - 3, k);
+            if constexpr (I == 0) {
+                r.up(W, 16 * (P - PC) + 1, H - 3, k);
                 r.stage_a(W - 1 - 4 * (H - 1), k);
             } else {
-                r.up(W, 16 * (P - PC) + 1, 16 * (P - PC) + 15, k);
+                r.up(W, 16 * (P - PC) + 1, 16 * (P - PREV) - 1, k);
             }
             r.template save<W>(img, lane);
             wave_sync();
-            // later regions lie inside region PC - 1: ring PC is final
-            rings.template store<PC>(img, out, n);
-            if constexpr (PC == 2) rings.template store<1>(img, out, n);
+            // later regions lie inside region PREV: rings PREV + 1 .. PC are final
+            for_range<PREV + 1, PC>([&](auto j) { rings.template store<decltype(j)::value>(img, out, n); });
         };
-        using Regions = std::make_integer_sequence<int, P - 2>;  // i -> PC = i + 2
+        using Regions = std::make_integer_sequence<int, K>;
         if (TILE != TILE_UP) {
             if constexpr (!RING_IN) {
                 full.template save<W>(img, lane);
                 wave_sync();
             }
-            for_each_index([&](auto i) { down_region(std::integral_constant<int, decltype(i)::value + 2>{}); },
-                           Regions{});
+            for_each_index(down_region, Regions{});
+            if constexpr (RING_IN && RK + 2 <= P) {
+                for_range<RK + 2, P>([&](auto j) { rings.template put<decltype(j)::value>(img); });
+                wave_sync();
+            }
             reload_full();
-            full.down(16 * (P - 1) + 1, H - 1, k);
+            full.down(16 * RK + 1, H - 1, k);
         }
         if (TILE != TILE_DOWN) {
-            full.up(W, 1, 15, k);
+            full.up(W, 1, 16 * (P - RK) - 1, k);
             full.template save<W>(img, lane);
             wave_sync();
-            rings.template store<P>(img, out, n);
-            for_each_index([&](auto i) { up_region(std::integral_constant<int, P - 1 - decltype(i)::value>{}); },
-                           Regions{});
+            for_range<RK + 1, P>([&](auto j) { rings.template store<decltype(j)::value>(img, out, n); });
+            for_each_index(up_region, Regions{});
         }
     }
     if constexpr (!RING_OUT) {
@@ -725,9 +755,12 @@ struct EulerRegion {
     }
 };
 
+// Smallest region of the phased Euler tiles, in cells per lane.
+constexpr int kEuRegionFirst = 1;
+
 // Swept Euler tiles: one wave per node of W = 64*P cells.  For P >= 4 the
 // node's image sits in LDS and the diamond's levels run on the narrowest
-// centred region of 2, 3, .. P cells per lane that holds their active set and
+// centred region of 1, 2, .. P cells per lane that holds their active set and
 // the two-cell stencil reach.  Same operations, same bits.  An Euler launch
 // is long (its stages divide and take square roots), so the node's load and
 // store are a fraction of a percent of it: the node moves whole, not by rings,
@@ -763,10 +796,11 @@ __global__ __launch_bounds__(kBlock) void k_euler_swept(const T* __restrict__ in
             full.template stage<0>(W - 1 - 4 * (H - 1), k);
         }
     } else {
-        // Regions of PC = 2, 3, .. P-1 cells per lane (widths 64 PC): region PC
-        // takes down pairs 16 PC - 15 <= s <= 16 PC - 1 (from 1 for PC = 2) and
-        // up pairs 16(P-PC)+1 <= m <= 16(P-PC)+15 (to H-3 for PC = 2), where
-        // every threshold is <= 64 PC - 1.
+        // Regions of PC = F, F + 1, .. P-1 cells per lane (F = kEuRegionFirst,
+        // widths 64 PC): region PC takes down pairs 16 PC - 15 <= s <= 16 PC - 1
+        // (from 1 for PC = F) and up pairs 16(P-PC)+1 <= m <= 16(P-PC)+15 (to
+        // H-3 for PC = F), where every threshold is <= 64 PC - 1.
+        constexpr int F = kEuRegionFirst;
         T* img = img_all + (threadIdx.x >> 6) * 6 * W;
         const int lane = g.lane;
         auto reload_full = [&] {
@@ -779,7 +813,7 @@ __global__ __launch_bounds__(kBlock) void k_euler_swept(const T* __restrict__ in
             constexpr int PC = decltype(pc_tag)::value;
             EulerRegion<T, PC> r;
             r.template load<W>(img, lane, g.base, n);
-            r.down(PC == 2 ? 1 : 16 * PC - 15, 16 * PC - 1, k);
+            r.down(PC == F ? 1 : 16 * PC - 15, 16 * PC - 1, k);
             r.template save<W>(img, lane);
             wave_sync();
         };
@@ -787,8 +821,8 @@ __global__ __launch_bounds__(kBlock) void k_euler_swept(const T* __restrict__ in
             constexpr int PC = decltype(pc_tag)::value;
             EulerRegion<T, PC> r;
             r.template load<W>(img, lane, g.base, n);
-            if constexpr (PC == 2) {
-                r.up(W, 16 * (P - 2) + 1, H - 3, k);
+            if constexpr (PC == F) {
+                r.up(W, 16 * (P - F) + 1, H - 3, k);
                 r.template stage<0>(W - 1 - 4 * (H - 1), k);
             } else {
                 r.up(W, 16 * (P - PC) + 1, 16 * (P - PC) + 15, k);
@@ -796,11 +830,11 @@ __global__ __launch_bounds__(kBlock) void k_euler_swept(const T* __restrict__ in
             r.template save<W>(img, lane);
             wave_sync();
         };
-        using Regions = std::make_integer_sequence<int, P - 2>;  // i -> PC = i + 2
+        using Regions = std::make_integer_sequence<int, P - F>;  // i -> PC = i + F
         if (TILE != TILE_UP) {
             full.template save<W>(img, lane);
             wave_sync();
-            for_each_index([&](auto i) { down_region(std::integral_constant<int, decltype(i)::value + 2>{}); },
+            for_each_index([&](auto i) { down_region(std::integral_constant<int, decltype(i)::value + F>{}); },
                            Regions{});
             reload_full();
             full.down(16 * (P - 1) + 1, H - 1, k);

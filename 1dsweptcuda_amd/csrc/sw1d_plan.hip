@@ -608,7 +608,9 @@ void release(sw1d_plan* p)
     delete p;
 }
 
-int g_lane_check = 0;  // 0 unknown, 1 pass, -1 fail
+// Result of the DPP lane-shift self check: 0 not run yet, 1 passed, -1
+// failed.  Plans may be created from several host threads at once.
+std::atomic<int> g_lane_check{0};
 
 // Euler states: callers hold interleaved (rho, m, E) cells (the reference's
 // REALthree arrays), the plan three component arrays plus their predictor

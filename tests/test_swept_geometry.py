@@ -69,9 +69,9 @@ def test_emulator_detects_a_wrong_active_set(orc, monkeypatch):
 @pytest.mark.parametrize("P", [4, 8, 16])
 def test_ks_region_phases_contain_active_sets(P):
     """The KS diamond kernel (k_ks_swept, P >= 4) runs its stage levels on
-    centred regions of 2, 3, .. P-1 points per lane below the full width; the
-    same thresholds as Euler's two-cell stencil."""
-    test_euler_region_phases_contain_active_sets(P, tuple(range(2, P)))
+    centred regions of 1, 2, .. P-1 points per lane below the full width
+    (ks_region); the same thresholds as Euler's two-cell stencil."""
+    test_euler_region_phases_contain_active_sets(P, tuple(range(1, P)))
 
 
 @pytest.mark.parametrize("P", [2, 4, 8, 16])
@@ -106,10 +106,11 @@ def test_heat_region_phases_contain_active_sets(P):
             assert np.all(d[halo] > t), (pc, t)
 
 
-@pytest.mark.parametrize("P,pcs", [(4, (2, 3)), (8, (2, 3, 4, 5, 6, 7))])
+@pytest.mark.parametrize("P,pcs", [(4, (1, 2, 3)), (8, (1, 2, 3, 4, 5, 6, 7)), (8, (2, 3, 4, 5, 6, 7))])
 def test_euler_region_phases_contain_active_sets(P, pcs):
-    """k_euler_swept: two-cell stencil like KS; regions of PC = 2, 3, .. P-1
-    cells per lane below the full width."""
+    """k_euler_swept: two-cell stencil like KS; regions of PC = 1, 2, .. P-1
+    cells per lane below the full width (kEuRegionFirst = 1), and the
+    schedule from 2 that preceded it."""
     W, H = 64 * P, 16 * P
     S = [16 * pc - 1 for pc in pcs]
     M = [16 * (P - pc) + 1 for pc in pcs]
@@ -143,7 +144,7 @@ def _ring_x(W, pc, lane):
     return W // 2 - 32 * pc + lane if lane < 32 else W // 2 + 32 * (pc - 1) + (lane - 32)
 
 
-@pytest.mark.parametrize("P,first,reach", [(4, 2, 2), (8, 2, 2), (16, 2, 2),
+@pytest.mark.parametrize("P,first,reach", [(4, 1, 2), (8, 1, 2), (16, 1, 2),
                                            (2, 1, 1), (4, 1, 1), (16, 1, 1)])
 def test_rings_partition_the_regions(P, first, reach):
     """The swept tiles move a node between HBM and its LDS image ring by ring
@@ -152,7 +153,8 @@ def test_rings_partition_the_regions(P, first, reach):
     rings 1..pc are exactly region pc's points (so region pc reads only rings
     already in the image and writes no point of an outer ring), region pc's
     stencil halo lies in ring pc + 1, and rings 1..P cover the node once.
-    KS regions start at 2 points per lane (two-point reach), heat at 1."""
+    KS (two-point reach) and heat (one-point reach) regions start at 1 point
+    per lane."""
     W = 64 * P
     seen = []
     for pc in range(1, P + 1):
