@@ -726,130 +726,239 @@ struct EulerRegion {
             gNm1 = gl == n - 2;
         }
     }
+    // The region is the middle 64*PC cells of a node of W cells (all of it
+    // when PC = W/64): lane l holds node-local x0 = W/2 - 32 PC + l PC ..
+    // x0 + PC - 1, and the halo is the two cells beyond each end.
     template <int W>
-    __device__ __forceinline__ void load(const T* img, int lane, int64_t base, int64_t n)
+    static constexpr int xs() { return W / 2 - 32 * PC; }
+    // every component and the halo from the state arrays (node base `base`)
+    template <int W>
+    __device__ __forceinline__ void load_global(const T* in, int64_t n, int64_t base, int lane)
     {
-        constexpr int xs = W / 2 - 32 * PC;
-        static_assert(xs >= 2 && xs + 64 * PC + 2 <= W, "region and halo inside the node");
-        const int x0 = xs + lane * PC;
+        const int x0 = xs<W>() + lane * PC;
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
 #pragma unroll
-            for (int p = 0; p < PC; ++p) q[c][p] = img[c * W + x0 + p];
-#pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                hl[c][i] = img[c * W + xs - 1 - i];
-                hr[c][i] = img[c * W + xs + 64 * PC + i];
-            }
+            for (int p = 0; p < PC; ++p) q[c][p] = in[c * n + wrap(base + x0 + p, n)];
         }
-        place(W, x0, base, n);
+        halo_global<W>(in, n, base);
     }
     template <int W>
-    __device__ __forceinline__ void save(T* img, int lane) const
+    __device__ __forceinline__ void halo_global(const T* in, int64_t n, int64_t base)
     {
-        const int x0 = W / 2 - 32 * PC + lane * PC;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                hl[c][i] = in[c * n + wrap(base + xs<W>() - 1 - i, n)];
+                hr[c][i] = in[c * n + wrap(base + xs<W>() + 64 * PC + i, n)];
+            }
+        }
+    }
+    // Components C0 .. C0+2 to / from a three-component node image (component
+    // c at img + (c - C0) W, node-local cell order).
+    template <int W, int C0>
+    __device__ __forceinline__ void save_half(T* img, int lane) const
+    {
+        const int x0 = xs<W>() + lane * PC;
+#pragma unroll
+        for (int c = C0; c < C0 + 3; ++c)
+#pragma unroll
+            for (int p = 0; p < PC; ++p) img[(c - C0) * W + x0 + p] = q[c][p];
+    }
+    template <int W, int C0>
+    __device__ __forceinline__ void load_half(const T* img, int lane)
+    {
+        const int x0 = xs<W>() + lane * PC;
+#pragma unroll
+        for (int c = C0; c < C0 + 3; ++c)
+#pragma unroll
+            for (int p = 0; p < PC; ++p) q[c][p] = img[(c - C0) * W + x0 + p];
+    }
+    template <int W, int C0>
+    __device__ __forceinline__ void halo_half(const T* img)
+    {
+        static_assert(xs<W>() >= 2 && xs<W>() + 64 * PC + 2 <= W, "halo inside the node");
+#pragma unroll
+        for (int c = C0; c < C0 + 3; ++c) {
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                hl[c][i] = img[(c - C0) * W + xs<W>() - 1 - i];
+                hr[c][i] = img[(c - C0) * W + xs<W>() + 64 * PC + i];
+            }
+        }
+    }
+    // the region's cells to the state arrays
+    template <int W>
+    __device__ __forceinline__ void store_global(T* out, int64_t n, int64_t base, int lane) const
+    {
+        const int x0 = xs<W>() + lane * PC;
 #pragma unroll
         for (int c = 0; c < 6; ++c)
 #pragma unroll
-            for (int p = 0; p < PC; ++p) img[c * W + x0 + p] = q[c][p];
+            for (int p = 0; p < PC; ++p) out[c * n + wrap(base + x0 + p, n)] = q[c][p];
     }
 };
+
+// Ring PC's cells of components C0 .. C0+2: from the state arrays into a
+// three-component node image, and from the image to the state arrays.
+template <typename T, int W, int PC, int C0>
+__device__ __forceinline__ void eu_ring_in(T* img, const T* in, int64_t n, int64_t base, int lane)
+{
+    const int x = ring_x<W, PC>(lane);
+#pragma unroll
+    for (int c = C0; c < C0 + 3; ++c) img[(c - C0) * W + x] = in[c * n + wrap(base + x, n)];
+}
+template <typename T, int W, int PC, int C0>
+__device__ __forceinline__ void eu_ring_out(const T* img, T* out, int64_t n, int64_t base, int lane)
+{
+    const int x = ring_x<W, PC>(lane);
+#pragma unroll
+    for (int c = C0; c < C0 + 3; ++c) out[c * n + wrap(base + x, n)] = img[(c - C0) * W + x];
+}
 
 // Smallest region of the phased Euler tiles, in cells per lane.
 constexpr int kEuRegionFirst = 1;
 
+// A phased Euler tile's node: its base in the state arrays and the wave's
+// three-component LDS image (3 W values).
+template <typename T>
+struct EuNode {
+    T* img;
+    const T* in;
+    T* out;
+    int64_t n, base;
+    int lane;
+};
+
+// Region A -> region B = A + 1 cells per lane (B = W/64: the whole node), one
+// half of the state at a time through the image.  The cells of ring B, and
+// the halo of region B, lie outside every region so far, so they still hold
+// the tile's input.
+template <typename T, int W, int A, int B>
+__device__ __forceinline__ void eu_grow(const EulerRegion<T, A>& a, EulerRegion<T, B>& b, const EuNode<T>& d)
+{
+    for_each_index(
+        [&](auto h) {
+            constexpr int C0 = 3 * decltype(h)::value;
+            a.template save_half<W, C0>(d.img, d.lane);
+            eu_ring_in<T, W, B, C0>(d.img, d.in, d.n, d.base, d.lane);
+            wave_sync();
+            b.template load_half<W, C0>(d.img, d.lane);
+            wave_sync();
+        },
+        std::make_integer_sequence<int, 2>{});
+    b.template halo_global<W>(d.in, d.n, d.base);
+    b.place(W, EulerRegion<T, B>::template xs<W>() + d.lane * B, d.base, d.n);
+}
+
+// Region A -> region B = A - 1 cells per lane: ring A is final and goes to
+// the output; region B's halo lies in ring A and comes from the image.
+template <typename T, int W, int A, int B>
+__device__ __forceinline__ void eu_shrink(const EulerRegion<T, A>& a, EulerRegion<T, B>& b, const EuNode<T>& d)
+{
+    for_each_index(
+        [&](auto h) {
+            constexpr int C0 = 3 * decltype(h)::value;
+            a.template save_half<W, C0>(d.img, d.lane);
+            wave_sync();
+            eu_ring_out<T, W, A, C0>(d.img, d.out, d.n, d.base, d.lane);
+            b.template load_half<W, C0>(d.img, d.lane);
+            b.template halo_half<W, C0>(d.img);
+            wave_sync();
+        },
+        std::make_integer_sequence<int, 2>{});
+    b.place(W, EulerRegion<T, B>::template xs<W>() + d.lane * B, d.base, d.n);
+}
+
+// The down phase from region PC (loaded) on: region PC takes down pairs
+// 16 PC - 15 <= s <= 16 PC - 1 (from 1 for the first region), where every
+// threshold is <= 64 PC - 1; the whole node ends it in `full`.
+template <typename T, int P, int PC>
+__device__ __forceinline__ void eu_down(EulerRegion<T, PC>& r, EulerRegion<T, P>& full, const EuNode<T>& d,
+                                        const EulerCoef<T>& k)
+{
+    constexpr int W = kWave * P, H = W / 4;
+    r.down(PC == kEuRegionFirst ? 1 : 16 * PC - 15, 16 * PC - 1, k);
+    if constexpr (PC + 1 < P) {
+        EulerRegion<T, PC + 1> r2;
+        eu_grow<T, W, PC, PC + 1>(r, r2, d);
+        eu_down<T, P, PC + 1>(r2, full, d, k);
+    } else {
+        eu_grow<T, W, PC, P>(r, full, d);
+        full.down(16 * (P - 1) + 1, H - 1, k);
+    }
+}
+
+// The up phase from region PC + 1 (or the whole node, PC + 1 = P) down:
+// region PC takes up pairs 16(P-PC)+1 <= m <= 16(P-PC)+15 (to H-3 for the
+// first region); each ring goes to the output once final, the first region
+// last.
+template <typename T, int P, int PC>
+__device__ __forceinline__ void eu_up(const EulerRegion<T, PC + 1>& a, const EuNode<T>& d, const EulerCoef<T>& k)
+{
+    constexpr int W = kWave * P, H = W / 4;
+    EulerRegion<T, PC> b;
+    eu_shrink<T, W, PC + 1, PC>(a, b, d);
+    if constexpr (PC == kEuRegionFirst) {
+        b.up(W, 16 * (P - PC) + 1, H - 3, k);
+        b.template stage<0>(W - 1 - 4 * (H - 1), k);
+        b.template store_global<W>(d.out, d.n, d.base, d.lane);
+    } else {
+        b.up(W, 16 * (P - PC) + 1, 16 * (P - PC) + 15, k);
+        eu_up<T, P, PC - 1>(b, d, k);
+    }
+}
+
 // Swept Euler tiles: one wave per node of W = 64*P cells.  For P >= 4 the
-// node's image sits in LDS and the diamond's levels run on the narrowest
-// centred region of 1, 2, .. P cells per lane that holds their active set and
-// the two-cell stencil reach.  Same operations, same bits.  An Euler launch
-// is long (its stages divide and take square roots), so the node's load and
-// store are a fraction of a percent of it: the node moves whole, not by rings,
-// which keeps the ring registers out of a kernel that already uses all 256.
+// diamond's levels run on the narrowest centred region of 1, 2, .. P cells
+// per lane that holds their active set and the two-cell stencil reach (same
+// operations, same bits), and a change of region moves the lane's cells
+// through an LDS image of three of the six components at a time: 12 KB per
+// wave at W = 512, so that two waves fit on a SIMD (the kernel is held to
+// 256 registers for the same reason).  The node's cells come from the input
+// ring by ring as the regions grow and go to the output ring by ring as they
+// shrink.
 template <typename T, int P, int TILE>
-__global__ __launch_bounds__(kBlock) void k_euler_swept(const T* __restrict__ in,
-                                                        T* __restrict__ out, int64_t n,
-                                                        int64_t nnodes, int64_t node_first,
-                                                        int64_t offset, EulerCoef<T> k)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2)))
+void k_euler_swept(const T* __restrict__ in, T* __restrict__ out, int64_t n, int64_t nnodes,
+                   int64_t node_first, int64_t offset, EulerCoef<T> k)
 {
     constexpr int W = kWave * P, H = W / 4;
     constexpr bool PHASED = P >= 4;
-    __shared__ T img_all[PHASED ? kWavesPerBlock * 6 * W : 1];
+    __shared__ T img_all[PHASED ? kWavesPerBlock * 3 * W : 1];
     NodeGeom g;
     if (!node_geom<P>(n, nnodes, node_first, offset, g)) return;
 
     EulerRegion<T, P> full;
-#pragma unroll
-    for (int c = 0; c < 6; ++c) {
-        load_pts<T, P>(in + c * n, g.lb, full.q[c]);
-#pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            full.hl[c][i] = in[c * n + wrap(g.base - 1 - i, n)];
-            full.hr[c][i] = in[c * n + wrap(g.base + W + i, n)];
-        }
-    }
-    full.place(W, g.lane * P, g.base, n);
-
     if constexpr (!PHASED) {
+        full.template load_global<W>(in, n, g.base, g.lane);
+        full.place(W, g.lane * P, g.base, n);
         if (TILE != TILE_UP) full.down(1, H - 1, k);
         if (TILE != TILE_DOWN) {
             full.up(W, 1, H - 3, k);
             full.template stage<0>(W - 1 - 4 * (H - 1), k);
         }
+        full.template store_global<W>(out, n, g.base, g.lane);
     } else {
-        // Regions of PC = F, F + 1, .. P-1 cells per lane (F = kEuRegionFirst,
-        // widths 64 PC): region PC takes down pairs 16 PC - 15 <= s <= 16 PC - 1
-        // (from 1 for PC = F) and up pairs 16(P-PC)+1 <= m <= 16(P-PC)+15 (to
-        // H-3 for PC = F), where every threshold is <= 64 PC - 1.
         constexpr int F = kEuRegionFirst;
-        T* img = img_all + (threadIdx.x >> 6) * 6 * W;
-        const int lane = g.lane;
-        auto reload_full = [&] {
-#pragma unroll
-            for (int c = 0; c < 6; ++c)
-#pragma unroll
-                for (int p = 0; p < P; ++p) full.q[c][p] = img[c * W + lane * P + p];
-        };
-        auto down_region = [&](auto pc_tag) {
-            constexpr int PC = decltype(pc_tag)::value;
-            EulerRegion<T, PC> r;
-            r.template load<W>(img, lane, g.base, n);
-            r.down(PC == F ? 1 : 16 * PC - 15, 16 * PC - 1, k);
-            r.template save<W>(img, lane);
-            wave_sync();
-        };
-        auto up_region = [&](auto pc_tag) {
-            constexpr int PC = decltype(pc_tag)::value;
-            EulerRegion<T, PC> r;
-            r.template load<W>(img, lane, g.base, n);
-            if constexpr (PC == F) {
-                r.up(W, 16 * (P - F) + 1, H - 3, k);
-                r.template stage<0>(W - 1 - 4 * (H - 1), k);
-            } else {
-                r.up(W, 16 * (P - PC) + 1, 16 * (P - PC) + 15, k);
-            }
-            r.template save<W>(img, lane);
-            wave_sync();
-        };
-        using Regions = std::make_integer_sequence<int, P - F>;  // i -> PC = i + F
+        const EuNode<T> d{img_all + (threadIdx.x >> 6) * 3 * W, in, out, n, g.base, g.lane};
         if (TILE != TILE_UP) {
-            full.template save<W>(img, lane);
-            wave_sync();
-            for_each_index([&](auto i) { down_region(std::integral_constant<int, decltype(i)::value + F>{}); },
-                           Regions{});
-            reload_full();
-            full.down(16 * (P - 1) + 1, H - 1, k);
+            EulerRegion<T, F> r;
+            r.template load_global<W>(in, n, g.base, g.lane);
+            r.place(W, EulerRegion<T, F>::template xs<W>() + g.lane * F, g.base, n);
+            eu_down<T, P, F>(r, full, d, k);
+        } else {
+            full.template load_global<W>(in, n, g.base, g.lane);
+            full.place(W, g.lane * P, g.base, n);
         }
         if (TILE != TILE_DOWN) {
             full.up(W, 1, 15, k);
-            full.template save<W>(img, lane);
-            wave_sync();
-            for_each_index([&](auto i) { up_region(std::integral_constant<int, P - 1 - decltype(i)::value>{}); },
-                           Regions{});
-            reload_full();
+            eu_up<T, P, P - 1>(full, d, k);
+        } else {
+            full.template store_global<W>(out, n, g.base, g.lane);
         }
     }
-#pragma unroll
-    for (int c = 0; c < 6; ++c) store_pts<T, P>(out + c * n, g.lb, full.q[c]);
 }
 
 // ------------------------------------------------------ classic kernels ---
