@@ -150,36 +150,40 @@ struct HeatRegion {
             stage_bc<false>(thr, src, hl, hr, dst, fo, fo2);
     }
     // down pairs s = s0, s0+2, .. s1 (thresholds 2s-1, 2s+1); up pairs
-    // m = m0, .. m1 (thresholds W-1-2m, W-3-2m)
-    template <bool MB>
+    // m = m0, .. m1 (thresholds W-1-2m, W-3-2m); U pairs per loop trip
+    template <bool MB, int U>
     __device__ __forceinline__ void down_bc(int s0, int s1, T fo, T fo2)
     {
+#pragma unroll U
         for (int s = s0; s <= s1; s += 2) {
             stage_bc<MB>(2 * s - 1, r0, hl0, hr0, r1, fo, fo2);
             stage_bc<MB>(2 * s + 1, r1, hl1, hr1, r0, fo, fo2);
         }
     }
-    template <bool MB>
+    template <bool MB, int U>
     __device__ __forceinline__ void up_bc(int W, int m0, int m1, T fo, T fo2)
     {
+#pragma unroll U
         for (int m = m0; m <= m1; m += 2) {
             stage_bc<MB>(W - 1 - 2 * m, r0, hl0, hr0, r1, fo, fo2);
             stage_bc<MB>(W - 3 - 2 * m, r1, hl1, hr1, r0, fo, fo2);
         }
     }
+    template <int U>
     __device__ __forceinline__ void down(int s0, int s1, T fo, T fo2)
     {
         if (MIRROR && bnd)
-            down_bc<MIRROR>(s0, s1, fo, fo2);
+            down_bc<MIRROR, U>(s0, s1, fo, fo2);
         else
-            down_bc<false>(s0, s1, fo, fo2);
+            down_bc<false, U>(s0, s1, fo, fo2);
     }
+    template <int U>
     __device__ __forceinline__ void up(int W, int m0, int m1, T fo, T fo2)
     {
         if (MIRROR && bnd)
-            up_bc<MIRROR>(W, m0, m1, fo, fo2);
+            up_bc<MIRROR, U>(W, m0, m1, fo, fo2);
         else
-            up_bc<false>(W, m0, m1, fo, fo2);
+            up_bc<false, U>(W, m0, m1, fo, fo2);
     }
     // flags and distance codes for a lane whose first point is node-local x0
     __device__ __forceinline__ void place(int W, int x0, int64_t base, int64_t n)
@@ -315,6 +319,10 @@ __global__ __launch_bounds__(kBlock) void k_heat_swept(const T* __restrict__ in,
     constexpr bool PHASED = P >= 2;
     constexpr bool RING_IN = PHASED && TILE != TILE_UP;
     constexpr bool RING_OUT = PHASED && TILE != TILE_DOWN;
+    // level pairs per loop trip: a heat level is three FP64 operations per
+    // point, so the loop's counter and branch are a visible share of it at up
+    // to 8 points per lane (+8-13% at W = 128 .. 512), none at 16
+    constexpr int U = P <= 8 ? 8 : 1;
     __shared__ T img_all[PHASED ? kWavesPerBlock * 2 * W : 1];
     NodeGeom g;
     if (!node_geom<P>(n, nnodes, node_first, offset, g)) return;
@@ -335,9 +343,9 @@ __global__ __launch_bounds__(kBlock) void k_heat_swept(const T* __restrict__ in,
     full.place(W, g.lane * P, g.base, n);
 
     if constexpr (!PHASED) {
-        if (TILE != TILE_UP) full.down(1, H - 1, fo, fo2);
+        if (TILE != TILE_UP) full.template down<U>(1, H - 1, fo, fo2);
         if (TILE != TILE_DOWN) {
-            full.up(W, 1, H - 3, fo, fo2);
+            full.template up<U>(W, 1, H - 3, fo, fo2);
             full.stage(W - 1 - 2 * (H - 1), full.r0, full.hl0, full.hr0, full.r1, fo, fo2);
         }
     } else {
@@ -363,7 +371,7 @@ __global__ __launch_bounds__(kBlock) void k_heat_swept(const T* __restrict__ in,
             }
             HeatRegion<T, PC, MIRROR, XM> r;
             r.template load<W>(img, lane, g.base, n);
-            r.down(32 * (PC - 1) + 1, 32 * PC - 1, fo, fo2);
+            r.template down<U>(32 * (PC - 1) + 1, 32 * PC - 1, fo, fo2);
             r.template save<W>(img, lane);
             wave_sync();
         };
@@ -372,10 +380,10 @@ __global__ __launch_bounds__(kBlock) void k_heat_swept(const T* __restrict__ in,
             HeatRegion<T, PC, MIRROR, XM> r;
             r.template load<W>(img, lane, g.base, n);
             if constexpr (PC == 1) {
-                r.up(W, 32 * (P - PC) + 1, H - 3, fo, fo2);
+                r.template up<U>(W, 32 * (P - PC) + 1, H - 3, fo, fo2);
                 r.stage(W - 1 - 2 * (H - 1), r.r0, r.hl0, r.hr0, r.r1, fo, fo2);
             } else {
-                r.up(W, 32 * (P - PC) + 1, 32 * (P - PC + 1) - 1, fo, fo2);
+                r.template up<U>(W, 32 * (P - PC) + 1, 32 * (P - PC + 1) - 1, fo, fo2);
             }
             r.template save<W>(img, lane);
             wave_sync();
@@ -391,10 +399,10 @@ __global__ __launch_bounds__(kBlock) void k_heat_swept(const T* __restrict__ in,
             for_each_index([&](auto i) { down_region(std::integral_constant<int, decltype(i)::value + 1>{}); },
                            Regions{});
             reload_full();
-            full.down(32 * (P - 1) + 1, H - 1, fo, fo2);
+            full.template down<U>(32 * (P - 1) + 1, H - 1, fo, fo2);
         }
         if (TILE != TILE_DOWN) {
-            full.up(W, 1, 31, fo, fo2);
+            full.template up<U>(W, 1, 31, fo, fo2);
             full.template save<W>(img, lane);
             wave_sync();
             rings.template store<P>(img, out, n);
@@ -455,13 +463,18 @@ struct KsRegion {
         }
     }
     // Down-triangle pairs s = s0, s0+2, .. s1 (thresholds 4s-1, 4s+3) and
-    // up-triangle pairs m = m0, m0+2, .. m1 (thresholds W-1-4m, W-5-4m).
+    // up-triangle pairs m = m0, m0+2, .. m1 (thresholds W-1-4m, W-5-4m), U
+    // pairs per loop trip.
+    template <int U>
     __device__ __forceinline__ void down(int s0, int s1, const KsCoef<T>& k)
     {
+#pragma unroll U
         for (int s = s0; s <= s1; s += 2) { stage_a(4 * s - 1, k); stage_b(4 * s + 3, k); }
     }
+    template <int U>
     __device__ __forceinline__ void up(int W, int m0, int m1, const KsCoef<T>& k)
     {
+#pragma unroll U
         for (int m = m0; m <= m1; m += 2) { stage_a(W - 1 - 4 * m, k); stage_b(W - 5 - 4 * m, k); }
     }
     // Load the middle 64*PC points of the node image (u at img[0..W), u_half
@@ -533,6 +546,10 @@ __global__ __launch_bounds__(kBlock) void k_ks_swept(const T* __restrict__ in,
     constexpr bool PHASED = P >= 4;
     constexpr bool RING_IN = PHASED && TILE != TILE_UP;
     constexpr bool RING_OUT = PHASED && TILE != TILE_DOWN;
+    // level pairs per loop trip: the loop's counter and branch are a visible
+    // share of a pair at up to 8 points per lane (+6-20% at W = 64 .. 512),
+    // none at 16
+    constexpr int U = P <= 8 ? 4 : 1;
     __shared__ T img_all[PHASED ? kWavesPerBlock * 2 * W : 1];
     NodeGeom g;
     if (!node_geom<P>(n, nnodes, node_first, offset, g)) return;
@@ -556,9 +573,9 @@ __global__ __launch_bounds__(kBlock) void k_ks_swept(const T* __restrict__ in,
     for (int p = 0; p < P; ++p) full.d[p] = abs(2 * (g.lane * P + p) - (W - 1));
 
     if constexpr (!PHASED) {
-        if (TILE != TILE_UP) full.down(1, H - 1, k);
+        if (TILE != TILE_UP) full.template down<U>(1, H - 1, k);
         if (TILE != TILE_DOWN) {
-            full.up(W, 1, H - 3, k);
+            full.template up<U>(W, 1, H - 3, k);
             full.stage_a(W - 1 - 4 * (H - 1), k);
         }
     } else {
@@ -590,7 +607,7 @@ __global__ __launch_bounds__(kBlock) void k_ks_swept(const T* __restrict__ in,
             }
             KsRegion<T, PC, XM> r;
             r.template load<W>(img, lane);
-            r.down(I == 0 ? 1 : 16 * PREV + 1, 16 * PC - 1, k);
+            r.template down<U>(I == 0 ? 1 : 16 * PREV + 1, 16 * PC - 1, k);
             r.template save<W>(img, lane);
             wave_sync();
         };
@@ -600,10 +617,10 @@ __global__ __launch_bounds__(kBlock) void k_ks_swept(const T* __restrict__ in,
             KsRegion<T, PC, XM> r;
             r.template load<W>(img, lane);
             if constexpr (I == 0) {
-                r.up(W, 16 * (P - PC) + 1, H - 3, k);
+                r.template up<U>(W, 16 * (P - PC) + 1, H - 3, k);
                 r.stage_a(W - 1 - 4 * (H - 1), k);
             } else {
-                r.up(W, 16 * (P - PC) + 1, 16 * (P - PREV) - 1, k);
+                r.template up<U>(W, 16 * (P - PC) + 1, 16 * (P - PREV) - 1, k);
             }
             r.template save<W>(img, lane);
             wave_sync();
@@ -622,10 +639,10 @@ __global__ __launch_bounds__(kBlock) void k_ks_swept(const T* __restrict__ in,
                 wave_sync();
             }
             reload_full();
-            full.down(16 * RK + 1, H - 1, k);
+            full.template down<U>(16 * RK + 1, H - 1, k);
         }
         if (TILE != TILE_DOWN) {
-            full.up(W, 1, 16 * (P - RK) - 1, k);
+            full.template up<U>(W, 1, 16 * (P - RK) - 1, k);
             full.template save<W>(img, lane);
             wave_sync();
             for_range<RK + 1, P>([&](auto j) { rings.template store<decltype(j)::value>(img, out, n); });
