@@ -13,8 +13,14 @@
 
 namespace sw1d {
 
+// i mod n in [0, n).  The tiles' indices lie within one period of the
+// domain (window starts in [0, n), windows and output ranges no longer than
+// n), so one add or subtract wraps them; the 64-bit remainder, a long
+// software sequence on the GPU, is there for any other index and runs only
+// in a wave that has one.
 __device__ __forceinline__ int64_t wrapn(int64_t i, int64_t n)
 {
+    if (i >= -n && i < 2 * n) return i < 0 ? i + n : (i >= n ? i - n : i);
     i %= n;
     return i < 0 ? i + n : i;
 }
