@@ -78,6 +78,7 @@ struct sw1d_plan {
     hipStream_t copy_stream = nullptr;
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
     void* host_win = nullptr;  // pinned (W + 2R) * nc REALs
+    std::vector<unsigned char> host_flux;  // the host node's interface fluxes: 2 (W + 1) Q3<REAL>
 
     // profiling
     bool profiling = false;
@@ -396,6 +397,21 @@ void cpu_node_diamond(sw1d_plan* p, T* win)
             const int src = st == 0 ? 0 : 3, dst = 3 - src;
             const T *s0 = row(src), *s1 = row(src + 1), *s2 = row(src + 2);
             T *d0 = row(dst), *d1 = row(dst + 1), *d2 = row(dst + 2);
+            auto cell = [&](int x) { return Q3<T>{s0[x], s1[x], s2[x]}; };
+            // Each interface x+1/2 (x = lo-1 .. hi) once, for the two cells beside it:
+            // the values eu_net_flux gives each cell (Euler:307-389), the interface
+            // right of the ghost cell 0 dropping pdiff(-1,0) and the one left of
+            // the ghost cell n-1 dropping pdiff(n-1,n) (the flags of Euler:405, :424).
+            Q3<T>* F = reinterpret_cast<Q3<T>*>(p->host_flux.data());
+            Q3<T>* D = F + (W + 1);
+            for (int x = lo - 1; x <= hi; ++x) {
+                const T q0 = x == xg0 ? T(0) : eu_pdiff(cell(x - 1), cell(x));
+                const T q1 = eu_pdiff(cell(x), cell(x + 1));
+                const T q2 = x == xgN - 1 ? T(0) : eu_pdiff(cell(x + 1), cell(x + 2));
+                eu_iface(cell(x - 1), cell(x), cell(x + 1), cell(x + 2), q0, q1, q2, k, F[x - lo + 1],
+                         D[x - lo + 1]);
+            }
+            const T c = st == 0 ? T(0.25) * k.dtdx : T(0.5) * k.dtdx;
             for (int x = lo; x <= hi; ++x) {
                 Q3<T> q;
                 if (x == xg0) {
@@ -403,13 +419,8 @@ void cpu_node_diamond(sw1d_plan* p, T* win)
                 } else if (x == xgN) {
                     q = k.br;
                 } else {
-                    Q3<T> c[5];
-                    for (int j = 0; j < 5; ++j) c[j] = Q3<T>{s0[x - 2 + j], s1[x - 2 + j], s2[x - 2 + j]};
-                    const bool fl = x == xg0 + 1, fr = x == xgN - 1;
-                    if (st == 0)
-                        q = eu_predict(c[0], c[1], c[2], c[3], c[4], fl, fr, k);
-                    else
-                        q = eu_correct(Q3<T>{d0[x], d1[x], d2[x]}, c[0], c[1], c[2], c[3], c[4], fl, fr, k);
+                    const Q3<T> f = eu_balance(F[x - lo], D[x - lo], F[x - lo + 1], D[x - lo + 1]);
+                    q = eu_apply(st == 0 ? cell(x) : Q3<T>{d0[x], d1[x], d2[x]}, f, c);
                 }
                 d0[x] = q.x;
                 d1[x] = q.y;
@@ -837,6 +848,7 @@ int sw1d_plan_create(sw1d_plan** out, const sw1d_problem* q)
     }
     if (p->alg == SW1D_ALT && p->eq != EQ_KS) {
         const size_t wb = (size_t)(p->W + 2 * p->R) * p->nc * p->S;
+        p->host_flux.assign((size_t)2 * (p->W + 1) * 3 * p->S, 0);
         if (hipHostMalloc(&p->host_win, wb, hipHostMallocDefault) != hipSuccess ||
             hipStreamCreateWithFlags(&p->copy_stream, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&p->ev_in, hipEventDisableTiming) != hipSuccess ||
