@@ -1047,33 +1047,48 @@ __global__ __launch_bounds__(kClassicBlock) void k_ks_classic(T* __restrict__ bu
     }
 }
 
+// Euler classic stage (Euler:391-426) with each interface flux evaluated once:
+// lane l of a wave computes the interface between cells x = 63 w + l - 1 and
+// x + 1, and (l < 63) updates cell x + 1 from it and from lane l + 1's,
+// which a DPP shift brings: 63 cells per wave, and the values the per-cell
+// form computes twice, so the same bits.  Cell indices are clamped at the
+// domain ends as in the per-cell form (the values so read are unused there).
 template <typename T, int STAGE>
 __global__ __launch_bounds__(kClassicBlock) void k_euler_classic(T* __restrict__ buf, int64_t n,
                                                                  EulerCoef<T> k)
 {
-    const int64_t i = (int64_t)blockIdx.x * kClassicBlock + threadIdx.x;
-    if (i >= n) return;
     constexpr int src = STAGE == 0 ? 0 : 3, dst = 3 - src;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t wave = (int64_t)blockIdx.x * (kClassicBlock / kWave) + (threadIdx.x >> 6);
+    const int64_t x = wave * (kWave - 1) + lane - 1;
+    auto cell = [&](int64_t j) {
+        j = j < 0 ? 0 : (j > n - 1 ? n - 1 : j);
+        return Q3<T>{buf[(src + 0) * n + j], buf[(src + 1) * n + j], buf[(src + 2) * n + j]};
+    };
+    const Q3<T> A = cell(x - 1), B = cell(x), C = cell(x + 1), Dc = cell(x + 2);
+    // the interface right of cell 0 drops pdiff(-1,0), the one left of cell
+    // n-1 drops pdiff(n-1,n) (the flags of Euler:405, :424)
+    const T q0 = x == 0 ? T(0) : eu_pdiff(A, B);
+    const T q1 = eu_pdiff(B, C);
+    const T q2 = x == n - 2 ? T(0) : eu_pdiff(C, Dc);
+    Q3<T> FL, DL;
+    eu_iface(A, B, C, Dc, q0, q1, q2, k, FL, DL);
+    // every lane takes part in the shifts before any leaves
+    const Q3<T> FR{from_right(FL.x, FL.x), from_right(FL.y, FL.y), from_right(FL.z, FL.z)};
+    const Q3<T> DR{from_right(DL.x, DL.x), from_right(DL.y, DL.y), from_right(DL.z, DL.z)};
+    const int64_t i = x + 1;
+    if (lane == kWave - 1 || i >= n) return;
     Q3<T> nv;
     if (i == 0) {
         nv = k.bl;
     } else if (i == n - 1) {
         nv = k.br;
     } else {
-        Q3<T> s[5];
-#pragma unroll
-        for (int j = 0; j < 5; ++j) {
-            int64_t idx = i - 2 + j;
-            idx = idx < 0 ? 0 : (idx > n - 1 ? n - 1 : idx);
-            s[j] = Q3<T>{buf[(src + 0) * n + idx], buf[(src + 1) * n + idx], buf[(src + 2) * n + idx]};
-        }
-        const bool fl = i == 1, fr = i == n - 2;
-        if (STAGE == 0) {
-            nv = eu_predict(s[0], s[1], s[2], s[3], s[4], fl, fr, k);
-        } else {
-            const Q3<T> u{buf[i], buf[n + i], buf[2 * n + i]};
-            nv = eu_correct(u, s[0], s[1], s[2], s[3], s[4], fl, fr, k);
-        }
+        const Q3<T> f = eu_balance(FL, DL, FR, DR);
+        if (STAGE == 0)
+            nv = eu_apply(C, f, T(0.25) * k.dtdx);
+        else
+            nv = eu_apply(Q3<T>{buf[i], buf[n + i], buf[2 * n + i]}, f, T(0.5) * k.dtdx);
     }
     buf[(dst + 0) * n + i] = nv.x;
     buf[(dst + 1) * n + i] = nv.y;
@@ -1213,10 +1228,12 @@ hipError_t classic_prec(int eq, int bc, int stage, int64_t n, const void* in, vo
         }
     } else {
         T* b = static_cast<T*>(out);
+        const int64_t waves = (n + kWave - 2) / (kWave - 1);  // 63 cells per wave
+        const dim3 ge((unsigned)((waves + kClassicBlock / kWave - 1) / (kClassicBlock / kWave)));
         if (stage == 0)
-            hipLaunchKernelGGL((k_euler_classic<T, 0>), grid, dim3(kClassicBlock), 0, s, b, n, eu_of<T>(c));
+            hipLaunchKernelGGL((k_euler_classic<T, 0>), ge, dim3(kClassicBlock), 0, s, b, n, eu_of<T>(c));
         else
-            hipLaunchKernelGGL((k_euler_classic<T, 1>), grid, dim3(kClassicBlock), 0, s, b, n, eu_of<T>(c));
+            hipLaunchKernelGGL((k_euler_classic<T, 1>), ge, dim3(kClassicBlock), 0, s, b, n, eu_of<T>(c));
     }
     return hipGetLastError();
 }
