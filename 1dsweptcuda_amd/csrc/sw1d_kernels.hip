@@ -534,7 +534,7 @@ __host__ __device__ constexpr int ks_regions()
 // lane, each level on the narrowest region that holds its active set and the
 // two-point stencil reach.  Same operations on the same values as the
 // one-width form, so the same bits, with 47% fewer point evaluations at
-// P = 16 (44% at P = 8, 38% at P = 4).  The node moves between HBM and the
+// P = 16 (44% at P = 8, 37% at P = 4).  The node moves between HBM and the
 // image ring by ring (NodeRings).
 template <typename T, int P, int TILE, bool XM>
 __global__ __launch_bounds__(kBlock) void k_ks_swept(const T* __restrict__ in,
