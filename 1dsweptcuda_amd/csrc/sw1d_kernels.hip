@@ -32,6 +32,9 @@
 // Classic kernels — one thread per point, one launch per stage
 //   (classicHeat Heat:132-145, classicKS KS:189-207, classicEuler Euler:400-426).
 
+#include <type_traits>
+#include <utility>
+
 #include "sw1d_internal.hpp"
 #include "sw1d_lanes.hpp"
 
@@ -477,6 +480,100 @@ struct KsRegion {
 #pragma unroll U
         for (int m = m0; m <= m1; m += 2) { stage_a(W - 1 - 4 * m, k); stage_b(W - 5 - 4 * m, k); }
     }
+
+    // The masked stages with compile-time thresholds (FP32 diamonds at 8
+    // points per lane).  Lane l of a centred region holds d = |2(l PC + p) -
+    // 64 PC + 1| at point p, so each point's set of active lanes is known
+    // with the threshold: a point active in all 64 lanes takes its fma with
+    // no mask, a point active in none is not evaluated, and the points of
+    // one lane set run their fmas under one branch on the first such
+    // point's d, so EXEC is narrowed and restored once per set instead of
+    // once per point.  The fmas are inline asm, which the compiler keeps
+    // inside the branch; each is the final fma of stage_a / stage_b on the
+    // same values.
+    static constexpr uint64_t lane_set(int p, int thr)
+    {
+        uint64_t m = 0;
+        for (int l = 0; l < 64; ++l) {
+            const int v = 2 * (l * PC + p) - 64 * PC + 1;
+            if ((v < 0 ? -v : v) <= thr) m |= uint64_t(1) << l;
+        }
+        return m;
+    }
+    static constexpr int lane_set_first(int p, int thr)
+    {
+        for (int q = 0; q < p; ++q)
+            if (lane_set(q, thr) == lane_set(p, thr)) return q;
+        return p;
+    }
+    __device__ __forceinline__ static void fma_lanes(float& dst, float a, float b, float c)
+    {
+        asm volatile("v_fma_f32 %0, %1, %2, %3" : "+v"(dst) : "v"(a), "v"(b), "v"(c));
+    }
+    // dst[p] = fma(a, rhs(p), base(p)) in the lanes where point p is active
+    template <int THR, typename FR, typename FB>
+    __device__ __forceinline__ void put_sets(T (&dst)[PC], T a, FR&& rhs, FB&& base)
+    {
+        // the stage's own copy of the threshold: the compiler would otherwise
+        // merge equal compares of distant stages (the two halves of a diamond
+        // repeat thresholds) and keep their lane masks live in between
+        int thr = THR;
+        asm volatile("" : "+s"(thr));
+        T f[PC];
+        for_range<0, PC - 1>([&](auto pi) {
+            constexpr int p = decltype(pi)::value;
+            if constexpr (lane_set(p, THR) != 0) f[p] = rhs(p);
+        });
+        for_range<0, PC - 1>([&](auto pi) {
+            constexpr int p = decltype(pi)::value;
+            constexpr uint64_t m = lane_set(p, THR);
+            if constexpr (m == ~uint64_t(0)) {
+                dst[p] = fmadd(a, f[p], base(p));
+            } else if constexpr (m != 0 && lane_set_first(p, THR) == p) {
+                if (d[p] <= thr) {
+                    for_range<p, PC - 1>([&](auto qi) {
+                        constexpr int q = decltype(qi)::value;
+                        if constexpr (lane_set(q, THR) == m) fma_lanes(dst[q], a, f[q], base(q));
+                    });
+                }
+            }
+        });
+    }
+    // ks_predict / ks_correct: fma(-dt/2, F(u), u) into uh, fma(-dt, F(uh), u) into u
+    template <int THR>
+    __device__ __forceinline__ void stage_a_c(const KsCoef<T>& k)
+    {
+        T e[PC + 4];
+        gather_carry<T, PC, 2>(u, hlu, hru, e);
+        put_sets<THR>(uh, -k.dth, [&](int p) { return ks_rhs(e[p], e[p + 1], e[p + 2], e[p + 3], e[p + 4], k); },
+                      [&](int p) { return u[p]; });
+    }
+    template <int THR>
+    __device__ __forceinline__ void stage_b_c(const KsCoef<T>& k)
+    {
+        T e[PC + 4];
+        gather_carry<T, PC, 2>(uh, hlh, hrh, e);
+        put_sets<THR>(u, -k.dt, [&](int p) { return ks_rhs(e[p], e[p + 1], e[p + 2], e[p + 3], e[p + 4], k); },
+                      [&](int p) { return u[p]; });
+    }
+    template <int S0, int S1>
+    __device__ __forceinline__ void down_c(const KsCoef<T>& k)
+    {
+        if constexpr (S0 <= S1) {
+            stage_a_c<4 * S0 - 1>(k);
+            stage_b_c<4 * S0 + 3>(k);
+            down_c<S0 + 2, S1>(k);
+        }
+    }
+    template <int W, int M0, int M1>
+    __device__ __forceinline__ void up_c(const KsCoef<T>& k)
+    {
+        if constexpr (M0 <= M1) {
+            stage_a_c<W - 1 - 4 * M0>(k);
+            stage_b_c<W - 5 - 4 * M0>(k);
+            up_c<W, M0 + 2, M1>(k);
+        }
+    }
     // Load the middle 64*PC points of the node image (u at img[0..W), u_half
     // at img[W..2W)) with their two-point halos, which stay constant while
     // every stage's active set lies inside the region.
@@ -550,6 +647,10 @@ __global__ __launch_bounds__(kBlock) void k_ks_swept(const T* __restrict__ in,
     // share of a pair at up to 8 points per lane (+6-20% at W = 64 .. 512),
     // none at 16
     constexpr int U = P <= 8 ? 4 : 1;
+    // FP32 diamonds of the masked form at 8 points per lane run their stages
+    // unrolled with compile-time thresholds (KsRegion::down_c / up_c): +3.4%
+    // at 2^20; the same form measured -5% in FP64
+    constexpr bool CT = XM && PHASED && P == 8 && TILE == TILE_DIAMOND && std::is_same_v<T, float>;
     __shared__ T img_all[PHASED ? kWavesPerBlock * 2 * W : 1];
     NodeGeom g;
     if (!node_geom<P>(n, nnodes, node_first, offset, g)) return;
@@ -607,7 +708,10 @@ __global__ __launch_bounds__(kBlock) void k_ks_swept(const T* __restrict__ in,
             }
             KsRegion<T, PC, XM> r;
             r.template load<W>(img, lane);
-            r.template down<U>(I == 0 ? 1 : 16 * PREV + 1, 16 * PC - 1, k);
+            if constexpr (CT)
+                r.template down_c<I == 0 ? 1 : 16 * PREV + 1, 16 * PC - 1>(k);
+            else
+                r.template down<U>(I == 0 ? 1 : 16 * PREV + 1, 16 * PC - 1, k);
             r.template save<W>(img, lane);
             wave_sync();
         };
@@ -616,9 +720,14 @@ __global__ __launch_bounds__(kBlock) void k_ks_swept(const T* __restrict__ in,
             constexpr int PC = ks_region<P>(I), PREV = I == 0 ? 0 : ks_region<P>(I - 1);
             KsRegion<T, PC, XM> r;
             r.template load<W>(img, lane);
-            if constexpr (I == 0) {
+            if constexpr (I == 0 && CT) {
+                r.template up_c<W, 16 * (P - PC) + 1, H - 3>(k);
+                r.template stage_a_c<W - 1 - 4 * (H - 1)>(k);
+            } else if constexpr (I == 0) {
                 r.template up<U>(W, 16 * (P - PC) + 1, H - 3, k);
                 r.stage_a(W - 1 - 4 * (H - 1), k);
+            } else if constexpr (CT) {
+                r.template up_c<W, 16 * (P - PC) + 1, 16 * (P - PREV) - 1>(k);
             } else {
                 r.template up<U>(W, 16 * (P - PC) + 1, 16 * (P - PREV) - 1, k);
             }
@@ -639,10 +748,16 @@ __global__ __launch_bounds__(kBlock) void k_ks_swept(const T* __restrict__ in,
                 wave_sync();
             }
             reload_full();
-            full.template down<U>(16 * RK + 1, H - 1, k);
+            if constexpr (CT)
+                full.template down_c<16 * RK + 1, H - 1>(k);
+            else
+                full.template down<U>(16 * RK + 1, H - 1, k);
         }
         if (TILE != TILE_DOWN) {
-            full.template up<U>(W, 1, 16 * (P - RK) - 1, k);
+            if constexpr (CT)
+                full.template up_c<W, 1, 16 * (P - RK) - 1>(k);
+            else
+                full.template up<U>(W, 1, 16 * (P - RK) - 1, k);
             full.template save<W>(img, lane);
             wave_sync();
             for_range<RK + 1, P>([&](auto j) { rings.template store<decltype(j)::value>(img, out, n); });
