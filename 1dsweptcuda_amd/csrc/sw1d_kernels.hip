@@ -103,6 +103,68 @@ __device__ __forceinline__ void for_range(F&& f)
     }
 }
 
+// EXEC-masked stage fmas with a compile-time threshold.  Lane l of a centred
+// region of PC points per lane holds d = |2(l PC + p) - 64 PC + 1| at point
+// p, so each point's set of active lanes is known with the threshold: a
+// point active in all 64 lanes takes its fma with no mask, a point active in
+// none is not evaluated, and the points of one lane set run their fmas under
+// one branch on the first such point's d, so EXEC is narrowed and restored
+// once per set instead of once per point.  The fmas are inline asm, which
+// the compiler keeps inside the branch.
+template <int PC>
+struct LaneSets {
+    static constexpr uint64_t of(int p, int thr)
+    {
+        uint64_t m = 0;
+        for (int l = 0; l < 64; ++l) {
+            const int v = 2 * (l * PC + p) - 64 * PC + 1;
+            if ((v < 0 ? -v : v) <= thr) m |= uint64_t(1) << l;
+        }
+        return m;
+    }
+    static constexpr int first(int p, int thr)
+    {
+        for (int q = 0; q < p; ++q)
+            if (of(q, thr) == of(p, thr)) return q;
+        return p;
+    }
+};
+// (FP32 only: in FP64 the branches measured slower than one mask per point)
+__device__ __forceinline__ void fma_lanes(float& dst, float a, float b, float c)
+{
+    asm volatile("v_fma_f32 %0, %1, %2, %3" : "+v"(dst) : "v"(a), "v"(b), "v"(c));
+}
+// dst[p] = fma(a, rhs(p), base(p)) in the lanes where d[p] <= THR
+template <typename T, int PC, int THR, typename FR, typename FB>
+__device__ __forceinline__ void fma_lane_sets(T (&dst)[PC], const int (&d)[PC], T a, FR&& rhs, FB&& base)
+{
+    using S = LaneSets<PC>;
+    // the stage's own copy of the threshold: the compiler would otherwise
+    // merge equal compares of distant stages (the two halves of a diamond
+    // repeat thresholds) and keep their lane masks live in between
+    int thr = THR;
+    asm volatile("" : "+s"(thr));
+    T f[PC];
+    for_range<0, PC - 1>([&](auto pi) {
+        constexpr int p = decltype(pi)::value;
+        if constexpr (S::of(p, THR) != 0) f[p] = rhs(p);
+    });
+    for_range<0, PC - 1>([&](auto pi) {
+        constexpr int p = decltype(pi)::value;
+        constexpr uint64_t m = S::of(p, THR);
+        if constexpr (m == ~uint64_t(0)) {
+            dst[p] = fmadd(a, f[p], base(p));
+        } else if constexpr (m != 0 && S::first(p, THR) == p) {
+            if (d[p] <= thr) {
+                for_range<p, PC - 1>([&](auto qi) {
+                    constexpr int q = decltype(qi)::value;
+                    if constexpr (S::of(q, THR) == m) fma_lanes(dst[q], a, f[q], base(q));
+                });
+            }
+        }
+    });
+}
+
 // ----------------------------------------------------------- heat tiles ---
 // Two values per point, the even-level and the odd-level row (the two rows of
 // the reference's working array, Heat:151-164): a point just inside a
@@ -481,80 +543,26 @@ struct KsRegion {
         for (int m = m0; m <= m1; m += 2) { stage_a(W - 1 - 4 * m, k); stage_b(W - 5 - 4 * m, k); }
     }
 
-    // The masked stages with compile-time thresholds (FP32 diamonds at 8
-    // points per lane).  Lane l of a centred region holds d = |2(l PC + p) -
-    // 64 PC + 1| at point p, so each point's set of active lanes is known
-    // with the threshold: a point active in all 64 lanes takes its fma with
-    // no mask, a point active in none is not evaluated, and the points of
-    // one lane set run their fmas under one branch on the first such
-    // point's d, so EXEC is narrowed and restored once per set instead of
-    // once per point.  The fmas are inline asm, which the compiler keeps
-    // inside the branch; each is the final fma of stage_a / stage_b on the
-    // same values.
-    static constexpr uint64_t lane_set(int p, int thr)
-    {
-        uint64_t m = 0;
-        for (int l = 0; l < 64; ++l) {
-            const int v = 2 * (l * PC + p) - 64 * PC + 1;
-            if ((v < 0 ? -v : v) <= thr) m |= uint64_t(1) << l;
-        }
-        return m;
-    }
-    static constexpr int lane_set_first(int p, int thr)
-    {
-        for (int q = 0; q < p; ++q)
-            if (lane_set(q, thr) == lane_set(p, thr)) return q;
-        return p;
-    }
-    __device__ __forceinline__ static void fma_lanes(float& dst, float a, float b, float c)
-    {
-        asm volatile("v_fma_f32 %0, %1, %2, %3" : "+v"(dst) : "v"(a), "v"(b), "v"(c));
-    }
-    // dst[p] = fma(a, rhs(p), base(p)) in the lanes where point p is active
-    template <int THR, typename FR, typename FB>
-    __device__ __forceinline__ void put_sets(T (&dst)[PC], T a, FR&& rhs, FB&& base)
-    {
-        // the stage's own copy of the threshold: the compiler would otherwise
-        // merge equal compares of distant stages (the two halves of a diamond
-        // repeat thresholds) and keep their lane masks live in between
-        int thr = THR;
-        asm volatile("" : "+s"(thr));
-        T f[PC];
-        for_range<0, PC - 1>([&](auto pi) {
-            constexpr int p = decltype(pi)::value;
-            if constexpr (lane_set(p, THR) != 0) f[p] = rhs(p);
-        });
-        for_range<0, PC - 1>([&](auto pi) {
-            constexpr int p = decltype(pi)::value;
-            constexpr uint64_t m = lane_set(p, THR);
-            if constexpr (m == ~uint64_t(0)) {
-                dst[p] = fmadd(a, f[p], base(p));
-            } else if constexpr (m != 0 && lane_set_first(p, THR) == p) {
-                if (d[p] <= thr) {
-                    for_range<p, PC - 1>([&](auto qi) {
-                        constexpr int q = decltype(qi)::value;
-                        if constexpr (lane_set(q, THR) == m) fma_lanes(dst[q], a, f[q], base(q));
-                    });
-                }
-            }
-        });
-    }
-    // ks_predict / ks_correct: fma(-dt/2, F(u), u) into uh, fma(-dt, F(uh), u) into u
+    // The masked stages with compile-time thresholds (fma_lane_sets): the
+    // final fma of ks_predict / ks_correct, fma(-dt/2, F(u), u) into uh and
+    // fma(-dt, F(uh), u) into u, on the same values.
     template <int THR>
     __device__ __forceinline__ void stage_a_c(const KsCoef<T>& k)
     {
         T e[PC + 4];
         gather_carry<T, PC, 2>(u, hlu, hru, e);
-        put_sets<THR>(uh, -k.dth, [&](int p) { return ks_rhs(e[p], e[p + 1], e[p + 2], e[p + 3], e[p + 4], k); },
-                      [&](int p) { return u[p]; });
+        fma_lane_sets<T, PC, THR>(uh, d, -k.dth,
+                                  [&](int p) { return ks_rhs(e[p], e[p + 1], e[p + 2], e[p + 3], e[p + 4], k); },
+                                  [&](int p) { return u[p]; });
     }
     template <int THR>
     __device__ __forceinline__ void stage_b_c(const KsCoef<T>& k)
     {
         T e[PC + 4];
         gather_carry<T, PC, 2>(uh, hlh, hrh, e);
-        put_sets<THR>(u, -k.dt, [&](int p) { return ks_rhs(e[p], e[p + 1], e[p + 2], e[p + 3], e[p + 4], k); },
-                      [&](int p) { return u[p]; });
+        fma_lane_sets<T, PC, THR>(u, d, -k.dt,
+                                  [&](int p) { return ks_rhs(e[p], e[p + 1], e[p + 2], e[p + 3], e[p + 4], k); },
+                                  [&](int p) { return u[p]; });
     }
     template <int S0, int S1>
     __device__ __forceinline__ void down_c(const KsCoef<T>& k)

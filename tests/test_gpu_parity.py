@@ -261,10 +261,12 @@ def test_periodic_heat_configuration_c1(sw, orc, alg):
 # (equation, bc, precision, algorithm, node width, log2 nX): at least 2048
 # nodes, two per SIMD of MI355X's 1024, where the swept stages run their final
 # fma under an EXEC mask instead of a compare and select (KS FP32 at W = 512:
-# the diamond's stages with compile-time lane sets, one branch per set)
+# the diamond's stages with compile-time lane sets, one branch per set; heat
+# at its tuned W = 512 with both kinds of end and in the hybrid)
 XM_CASES = [(0, 0, 0, 1, 128, 18), (0, 2, 1, 1, 64, 17), (0, 0, 0, 2, 128, 18), (1, 0, 0, 1, 64, 17),
             (1, 0, 1, 1, 256, 19), (1, 0, 0, 1, 512, 20), (1, 0, 1, 1, 512, 20), (1, 0, 0, 2, 128, 18),
-            (1, 0, 1, 2, 128, 18)]
+            (1, 0, 1, 2, 128, 18), (0, 0, 0, 1, 512, 20), (0, 0, 1, 1, 512, 20), (0, 2, 0, 1, 512, 20),
+            (0, 0, 0, 2, 512, 20)]
 
 
 @pytest.mark.parametrize("eq,bc,prec,alg,W,lg", XM_CASES)
