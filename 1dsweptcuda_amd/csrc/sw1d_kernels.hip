@@ -656,8 +656,8 @@ __global__ __launch_bounds__(kBlock) void k_ks_swept(const T* __restrict__ in,
     // none at 16
     constexpr int U = P <= 8 ? 4 : 1;
     // FP32 diamonds of the masked form at 8 points per lane run their stages
-    // unrolled with compile-time thresholds (KsRegion::down_c / up_c): +3.4%
-    // at 2^20; the same form measured -5% in FP64
+    // unrolled with compile-time thresholds (KsRegion::down_c / up_c): +3.6%
+    // at 2^20, +8% at 2^21; the same form measured -5% in FP64
     constexpr bool CT = XM && PHASED && P == 8 && TILE == TILE_DIAMOND && std::is_same_v<T, float>;
     __shared__ T img_all[PHASED ? kWavesPerBlock * 2 * W : 1];
     NodeGeom g;
